@@ -1,0 +1,133 @@
+/*
+ * wlc_mde.h -- C ABI of libwlc_mde.so, the MI355X (gfx950) implementation of the
+ * wormlike-chain modified diffusion equation (MDE) propagator solve of
+ * tjz2026/wormlike-chain-triblock.
+ *
+ * The reference exposes the solve as Fortran module procedures that read and write
+ * module globals (module wormlike_MDE, wormlike_MDE_2dcomp_mod.f90:227-244, 961-973):
+ *   MDE_q(R2C_FFT)      forward propagator  qA, qB          (:231-956)
+ *   MDE_qstar(R2C_FFT)  backward propagator qAstar, qBstar  (:961-1557)
+ * with its inputs set beforehand by initialize/init_inverse_sparse_GAB and its
+ * outputs consumed by `density` (SCFT_Density_FE.f90:3-271).  Every entry point
+ * below replaces one of those pieces; the Fortran binding that keeps the reference
+ * names lives in wormlike-chain-triblock_amd/fortran/wormlike_mde_gpu.f90.
+ *
+ * Conventions: all arrays are FP64, fields are [nz][ny][nx] with x fastest (the
+ * reference's kD map, grid_mod.f90:356-379); block fields are stacked [nblk][...];
+ * propagator slices are [M][nz][ny][nx] with harmonic i <-> (l, m), i = l*l + l + m
+ * (SPH_mod.f90:214-222).  Host pointers are read/written synchronously; the handle
+ * owns all device memory.  Every function returns WLC_OK (0) or a negative status;
+ * nothing aborts.  A handle is not thread-safe; separate handles are independent.
+ */
+#ifndef WLC_MDE_H
+#define WLC_MDE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WLC_OK 0
+#define WLC_EINVAL (-1)       /* bad parameters or arguments */
+#define WLC_ENOMEM (-2)       /* device allocation failed */
+#define WLC_EHIP (-3)         /* a HIP runtime call failed */
+#define WLC_ESTATE (-4)       /* call out of order (e.g. density before the sweeps) */
+#define WLC_ENONFINITE (-5)   /* Q, totden or phi not finite */
+
+#define WLC_MAX_BLOCKS 3
+#define WLC_MAX_L 8
+
+typedef struct wlc_params {
+  int nx, ny, nz;                 /* grid; even edges with factors 2 and 3 only */
+  double box[3];                  /* box edges (x, y, z) in units of chain length */
+  int nblk;                       /* 1..3 blocks (A, B, C) */
+  int Ns;                         /* contour steps of the chain (reference Nmax); ds = 1/Ns */
+  int Nblk[WLC_MAX_BLOCKS];       /* contour steps per block (NA, NB, NC); sum = Ns */
+  int Lmax[WLC_MAX_BLOCKS];       /* harmonic cut-off per block, 0..WLC_MAX_L */
+  double lambda[WLC_MAX_BLOCKS];  /* angular coefficient Loa/(2 kappa) per block */
+  int keep_qstar;                 /* 1: keep every backward slice (for wlc_mde_get_q) */
+  int reserved[7];                /* must be zero */
+} wlc_params;
+
+typedef struct wlc_handle wlc_handle;
+
+/* Create a solver for one chain architecture and grid; builds the orientation
+ * tables and FFT plans and allocates the forward propagator store.
+ * Replaces: initialize's derived parameters (initialize_mod.f90:140-199), the
+ * allocations of init_arrays_2decomp (init_array_2dcomp_mod.f90:47-51) and
+ * decomp2d_mpi_3d_dft_r2c_init (2dcomp_fft_mod.f90:39-73). */
+int wlc_mde_create(const wlc_params* p, wlc_handle** out);
+
+/* Change the box.  Replaces init_inverse_sparse_GAB(R2C_FFT)
+ * (Gmatrix_2dcomp_mod.f90:222-650): there are no stored G^-1 tables, the k-aligned
+ * solve reads the box through its wave-number factors. */
+int wlc_mde_set_box(wlc_handle* h, const double box[3]);
+
+/* Set the block fields w [nblk][nz][ny][nx] from host / device memory.
+ * Replaces the module globals wA, wB (global_parameters.f90; ABC: w_blk). */
+int wlc_mde_set_w(wlc_handle* h, const double* w_host);
+int wlc_mde_set_w_device(wlc_handle* h, const double* w_dev);
+
+/* Forward propagator q(s), s = 0..Ns, q(0) = delta_{i0}.
+ * Replaces MDE_q(R2C_FFT) (wormlike_MDE_2dcomp_mod.f90:231-956). */
+int wlc_mde_forward(wlc_handle* h);
+
+/* Backward propagator q*(s), q*(Ns) = delta_{i0}; also accumulates the contour
+ * quadrature of sum_i q_i q*_i per block (needs a forward sweep on the same w).
+ * Replaces MDE_qstar(R2C_FFT) (wormlike_MDE_2dcomp_mod.f90:961-1557) and the
+ * contour loop of density (SCFT_Density_FE.f90:97-118). */
+int wlc_mde_backward(wlc_handle* h);
+
+/* Q, totden and phi [nblk][nz][ny][nx] (host or device pointer; phi may be NULL).
+ * Replaces density's Q / totden / phi part (SCFT_Density_FE.f90:48-159,
+ * SCFT_Density_FE_ABC.f90:98-139). */
+int wlc_mde_density(wlc_handle* h, double* phi_host, double* Q, double* totden);
+int wlc_mde_density_device(wlc_handle* h, double* phi_dev, double* Q, double* totden);
+
+/* Copy slice s of the forward (dir = +1) or backward (dir = -1) propagator to host
+ * memory as [M(s)][nz][ny][nx].  The backward store needs keep_qstar = 1, except for
+ * s = 0 which is always kept.  Replaces reading qA/qB/qAstar/qBstar
+ * (init_array_2dcomp_mod.f90:48-51) for tests and diagnostics. */
+int wlc_mde_get_q(wlc_handle* h, int dir, int s, double* q_host);
+
+/* harmonics held by slice s (M of the block that contains contour point s) */
+int wlc_mde_slice_M(const wlc_handle* h, int s);
+
+/* block index (0-based) that owns contour point s */
+int wlc_mde_block_of(const wlc_handle* h, int s);
+
+/* wait for all work queued on the handle's stream */
+int wlc_mde_sync(wlc_handle* h);
+
+/* the HIP stream (hipStream_t) all work of the handle is queued on */
+void* wlc_mde_stream(wlc_handle* h);
+
+/* Per-kernel timing with HIP events on the handle's stream.  While enabled, every
+ * launch of the step kernels is bracketed by events; wlc_mde_get_timing returns,
+ * for kernel k in {0: rhs+xy FFT, 1: z FFT+solve, 2: inverse xy FFT, 3: density
+ * accumulation}, the summed milliseconds and the launch count since enabling.
+ * No counterpart in the reference (which times only the whole run, SCFT_main.f90:51-55). */
+int wlc_mde_set_timing(wlc_handle* h, int enable);
+int wlc_mde_get_timing(wlc_handle* h, double* ms, long long* count, int nkernels);
+
+/* Event pair around queued work on the handle's stream; elapsed milliseconds
+ * between the two records.  No counterpart in the reference. */
+int wlc_mde_event_record(wlc_handle* h, int slot);
+int wlc_mde_event_elapsed(wlc_handle* h, int slot_a, int slot_b, double* ms);
+
+void wlc_mde_destroy(wlc_handle* h);
+
+/* message of the last failure on this thread ("" if none) */
+const char* wlc_mde_last_error(void);
+
+/* Test hook: the k-aligned solve of G(k) x = v for nk k-vectors on the device
+ * (kvec [nk][3] in wave-number units, v [nk][M] complex interleaved, in place).
+ * Replaces one G^-1 apply (4 x mkl_dcsrsymv, wormlike_MDE_2dcomp_mod.f90:395-409). */
+int wlc_test_solve(int L, int nk, const double* kvec_host, double cdelta, double sigma,
+                   double ds, double dslam, double* v_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+// Kernel launch interface of libwlc_mde.so (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "wlc_internal.h"
+
+namespace wlc {
+
+struct AxisDev {
+  int n, npass;
+  int radix[8];
+  const double2* tw;  // exp(-2 pi i t / n), t < n
+};
+
+struct Geom {
+  int nx, ny, nz, nxh;
+  AxisDev ax, ay, az;
+};
+
+// Kernel A: BDF right-hand side of one propagator step fused with the x (R2C, two
+// rows packed per complex transform) and y passes of the forward 3D FFT.
+struct RhsArgs {
+  const double* h[3];  // history slices [M_h][nz][ny][nx] (nullptr beyond the BDF order)
+  int hM[3];           // harmonics held by each history slice
+  double a[3], b[3];   // P = sum_j a_j q_j - ds w sum_j b_j q_j
+  double ds;
+  const double* w;     // field of the target block [nz][ny][nx]
+  int M;               // harmonics of the target slice
+  double2* spec;       // [M][nz][ny][nxh]
+};
+
+// Kernel B: z pass of the FFT, per-k solve of G(k) x = P(k), inverse z pass.
+struct SolveArgs {
+  double2* spec;
+  int M, L, TX;        // harmonics, Lmax, kx columns per workgroup
+  double cdelta, sigma, ds, dslam;
+  double kfac[3];      // 2 pi / box per axis
+  int solve;           // 0: FFT round trip only (test hook)
+};
+
+// Kernel C: inverse y and x (C2R) passes, scaling by 1/N, store of q(s).
+struct InvArgs {
+  const double2* spec;
+  int M;
+  double scale;
+  double* out;         // [M][nz][ny][nx]
+};
+
+hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, hipStream_t st);
+hipError_t launch_zsolve(const Geom& g, const SolveArgs& a, hipStream_t st);
+hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st);
+
+// small elementwise/reduction kernels
+hipError_t launch_set_delta(double* slice, int M, long long N, hipStream_t st);
+hipError_t launch_fill(double* dst, double v, long long n, hipStream_t st);
+// rho_t += wgt_t * sum_{i < lim_t} q_i qs_i for up to two targets (t = 0, 1)
+hipError_t launch_density_acc(const double* q, const double* qs, long long N, double* rho0,
+                              int lim0, double w0, double* rho1, int lim1, double w1,
+                              hipStream_t st);
+// partial[b] = sum over a chunk of r of W_r * f(r), f = sum_{t<nf} src[t*N + r]
+hipError_t launch_weighted_sum(const Geom& g, const double* src, int nf, double dV,
+                               double* partial, int nparts, double* result, hipStream_t st);
+// phi_b = V * rho_b / totden (totden read from device)
+hipError_t launch_phi(const double* rho, int nblk, long long N, double V, const double* totden,
+                      double* phi, hipStream_t st);
+
+hipError_t upload_orientation_tables(const OrientTables& t);
+
+// test hook: rotated-frame solve of nk vectors (M complex each, interleaved)
+hipError_t launch_test_solve(int L, int nk, const double* kvec, double cdelta, double sigma,
+                             double ds, double dslam, double2* v, hipStream_t st);
+
+}  // namespace wlc

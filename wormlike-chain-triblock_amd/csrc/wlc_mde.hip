@@ -1,0 +1,630 @@
+// C ABI of libwlc_mde.so: handle, propagator stores and the contour sweeps.
+//
+// Forward sweep (reference MDE_q, wormlike_MDE_2dcomp_mod.f90:231-956):
+//   q(0) = delta_{i0}; for s = 1..Ns: BDF order min(s, 3); the target point s takes
+//   the field, Lmax and lambda of the block that contains it (junction point = lower
+//   block, :587-591, :704); history slices of another block are zero-padded or
+//   truncated to the target block's harmonics (:727-734).
+// Backward sweep (reference MDE_qstar, :961-1557): mirror with sigma = -1, BDF order
+//   min(Ns - s, 3), q*(Ns) = delta_{i0}; while it runs, the per-block contour Simpson
+//   sum of sum_i q_i q*_i is accumulated (density, SCFT_Density_FE.f90:97-118).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wlc_mde.h"
+#include "wlc_internal.h"
+#include "wlc_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(WLC_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+std::mutex g_tab_mu;
+bool g_tab_built = false;
+wlc::OrientTables g_tab;
+std::vector<int> g_tab_uploaded;  // per device
+
+// composite Simpson over n intervals (n even: the reference's rule,
+// utility_mod.f90:346-374); odd n >= 3 closes with a 3/8 panel; n = 1 trapezoid
+void contour_weights(int n, double ds, std::vector<double>* w) {
+  w->assign(n + 1, 0.0);
+  if (n == 1) {
+    (*w)[0] = (*w)[1] = ds / 2.0;
+    return;
+  }
+  const int ns = (n % 2 == 0) ? n : n - 3;
+  for (int j = 0; j <= ns && ns > 0; ++j) {
+    const double c = (j == 0 || j == ns) ? 1.0 : ((j % 2) ? 4.0 : 2.0);
+    (*w)[j] += (ds / 3.0) * c;
+  }
+  if (n % 2) {
+    const double c = 3.0 * ds / 8.0;
+    (*w)[ns] += c;
+    (*w)[ns + 1] += 3.0 * c;
+    (*w)[ns + 2] += 3.0 * c;
+    (*w)[ns + 3] += c;
+  }
+}
+
+}  // namespace
+
+struct wlc_handle {
+  wlc_params p{};
+  int device = 0;
+  long long N = 0, Nk = 0;
+  int nxh = 0, Ns = 0, Mmax = 0;
+  double ds = 0.0;
+  int Mb[3] = {0, 0, 0}, Lb[3] = {0, 0, 0}, start[3] = {0, 0, 0}, TX[3] = {1, 1, 1};
+  std::vector<long long> off;  // forward store slice offsets (doubles)
+  std::vector<std::vector<double>> cw;  // contour weights per block
+  double* q = nullptr;
+  double* qs = nullptr;        // backward: full store (keep_qstar) or 4-slot ring
+  double2* spec = nullptr;
+  double* w = nullptr;
+  double* rho = nullptr;
+  double* phi = nullptr;
+  double* partial = nullptr;
+  double* scal = nullptr;      // [0] = int q_0(Ns), [1] = totden
+  int nparts = 256;
+  double2* tw[3] = {nullptr, nullptr, nullptr};
+  wlc::AxisPlan plan[3];
+  wlc::Geom geom{};
+  hipStream_t st = nullptr;
+  bool have_w = false, fwd_done = false, bwd_done = false;
+  // timing
+  bool timing = false;
+  struct Rec { int k; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double ms[4] = {0, 0, 0, 0};
+  long long cnt[4] = {0, 0, 0, 0};
+  hipEvent_t uev[8] = {};
+};
+
+namespace {
+
+int block_of(const wlc_handle* h, int s) {
+  for (int b = 0; b < h->p.nblk; ++b)
+    if (s <= h->start[b] + h->p.Nblk[b]) return b;
+  return h->p.nblk - 1;
+}
+
+double* qs_slot(wlc_handle* h, int s) {
+  if (h->p.keep_qstar) return h->qs + h->off[s];
+  return h->qs + (long long)(s & 3) * h->Mmax * h->N;
+}
+
+hipEvent_t take_event(wlc_handle* h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int flush_timing(wlc_handle* h) {
+  if (h->recs.empty()) return WLC_OK;
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (auto& r : h->recs) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+    h->ms[r.k] += t;
+    h->cnt[r.k] += 1;
+    h->pool.push_back(r.a);
+    h->pool.push_back(r.b);
+  }
+  h->recs.clear();
+  return WLC_OK;
+}
+
+struct TimedLaunch {
+  wlc_handle* h;
+  int k;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedLaunch(wlc_handle* hh, int kk) : h(hh), k(kk) {
+    if (h->timing) {
+      a = take_event(h);
+      b = take_event(h);
+      if (a) (void)hipEventRecord(a, h->st);
+    }
+  }
+  ~TimedLaunch() {
+    if (h->timing && a && b) {
+      (void)hipEventRecord(b, h->st);
+      h->recs.push_back({k, a, b});
+    }
+  }
+};
+
+int run_step(wlc_handle* h, int dir, int s) {
+  const int Ns = h->Ns;
+  const int order = dir > 0 ? std::min(s, 3) : std::min(Ns - s, 3);
+  const int blk = block_of(h, s);
+  const int M = h->Mb[blk];
+  wlc::RhsArgs ra{};
+  for (int j = 0; j < 3; ++j) {
+    const int sh = s - dir * (j + 1);
+    if (j < order && sh >= 0 && sh <= Ns) {
+      ra.h[j] = dir > 0 ? h->q + h->off[sh] : qs_slot(h, sh);
+      ra.hM[j] = h->Mb[block_of(h, sh)];
+    } else {
+      ra.h[j] = nullptr;
+      ra.hM[j] = 0;
+    }
+  }
+  double cdelta;
+  if (order == 1) {
+    ra.a[0] = 1.0; ra.a[1] = 0.0; ra.a[2] = 0.0;
+    ra.b[0] = 1.0; ra.b[1] = 0.0; ra.b[2] = 0.0;
+    cdelta = 1.0;
+  } else if (order == 2) {
+    ra.a[0] = 2.0; ra.a[1] = -0.5; ra.a[2] = 0.0;
+    ra.b[0] = 2.0; ra.b[1] = -1.0; ra.b[2] = 0.0;
+    cdelta = 1.5;
+  } else {
+    ra.a[0] = 3.0; ra.a[1] = -1.5; ra.a[2] = 1.0 / 3.0;
+    ra.b[0] = 3.0; ra.b[1] = -3.0; ra.b[2] = 1.0;
+    cdelta = 11.0 / 6.0;
+  }
+  ra.ds = h->ds;
+  ra.w = h->w + (long long)blk * h->N;
+  ra.M = M;
+  ra.spec = h->spec;
+  {
+    TimedLaunch t(h, 0);
+    HIPCHK(wlc::launch_rhs_fft(h->geom, ra, h->st));
+  }
+  wlc::SolveArgs sa{};
+  sa.spec = h->spec;
+  sa.M = M;
+  sa.L = h->Lb[blk];
+  sa.TX = h->TX[blk];
+  sa.cdelta = cdelta;
+  sa.sigma = dir > 0 ? 1.0 : -1.0;
+  sa.ds = h->ds;
+  sa.dslam = h->ds * h->p.lambda[blk];
+  const double tp = 2.0 * M_PI;
+  for (int a = 0; a < 3; ++a) sa.kfac[a] = tp / h->p.box[a];
+  sa.solve = 1;
+  {
+    TimedLaunch t(h, 1);
+    HIPCHK(wlc::launch_zsolve(h->geom, sa, h->st));
+  }
+  wlc::InvArgs ia{};
+  ia.spec = h->spec;
+  ia.M = M;
+  ia.scale = 1.0 / (double)h->N;
+  ia.out = dir > 0 ? h->q + h->off[s] : qs_slot(h, s);
+  {
+    TimedLaunch t(h, 2);
+    HIPCHK(wlc::launch_ifft_store(h->geom, ia, h->st));
+  }
+  return WLC_OK;
+}
+
+int density_at(wlc_handle* h, int s) {
+  double* rho[2] = {nullptr, nullptr};
+  int lim[2] = {0, 0};
+  double wt[2] = {0.0, 0.0};
+  int n = 0;
+  const int Ms = h->Mb[block_of(h, s)];
+  for (int b = 0; b < h->p.nblk && n < 2; ++b) {
+    const int j = s - h->start[b];
+    if (j < 0 || j > h->p.Nblk[b]) continue;
+    rho[n] = h->rho + (long long)b * h->N;
+    lim[n] = std::min(Ms, h->Mb[b]);
+    wt[n] = h->cw[b][j];
+    ++n;
+  }
+  if (n == 0) return WLC_OK;
+  TimedLaunch t(h, 3);
+  HIPCHK(wlc::launch_density_acc(h->q + h->off[s], qs_slot(h, s), h->N, rho[0], lim[0], wt[0],
+                                 rho[1], lim[1], wt[1], h->st));
+  return WLC_OK;
+}
+
+void free_handle(wlc_handle* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  for (auto& r : h->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  for (auto e : h->uev) if (e) (void)hipEventDestroy(e);
+  (void)hipFree(h->q);
+  (void)hipFree(h->qs);
+  (void)hipFree(h->spec);
+  (void)hipFree(h->w);
+  (void)hipFree(h->rho);
+  (void)hipFree(h->phi);
+  (void)hipFree(h->partial);
+  (void)hipFree(h->scal);
+  for (auto t : h->tw) (void)hipFree(t);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+}
+
+int validate(const wlc_params* p) {
+  if (!p) return fail(WLC_EINVAL, "params is NULL");
+  const int e[3] = {p->nx, p->ny, p->nz};
+  for (int a = 0; a < 3; ++a) {
+    if (e[a] < 2 || e[a] % 2) return fail(WLC_EINVAL, "grid edges must be even and >= 2");
+    int r = e[a];
+    while (r % 2 == 0) r /= 2;
+    while (r % 3 == 0) r /= 3;
+    if (r != 1) return fail(WLC_EINVAL, "grid edges must factor into 2s and 3s");
+    if (!(p->box[a] > 0.0) || !std::isfinite(p->box[a])) return fail(WLC_EINVAL, "box edges must be positive");
+  }
+  if (p->nx > 256 || p->ny > 256 || p->nz > 256) return fail(WLC_EINVAL, "grid edges above 256 are not supported");
+  if (p->nblk < 1 || p->nblk > WLC_MAX_BLOCKS) return fail(WLC_EINVAL, "nblk must be 1..3");
+  int tot = 0;
+  for (int b = 0; b < p->nblk; ++b) {
+    if (p->Nblk[b] < 1) return fail(WLC_EINVAL, "every block needs >= 1 contour step");
+    if (p->Lmax[b] < 0 || p->Lmax[b] > WLC_MAX_L) return fail(WLC_EINVAL, "Lmax must be 0..8");
+    if (!(p->lambda[b] >= 0.0) || !std::isfinite(p->lambda[b])) return fail(WLC_EINVAL, "lambda must be >= 0");
+    tot += p->Nblk[b];
+  }
+  if (tot != p->Ns) return fail(WLC_EINVAL, "sum of block lengths must equal Ns");
+  if (p->Ns < 3) return fail(WLC_EINVAL, "Ns must be >= 3");
+  for (int r : p->reserved)
+    if (r) return fail(WLC_EINVAL, "reserved fields must be zero");
+  return WLC_OK;
+}
+
+int pick_tx(int M, int nz, int nxh) {
+  const char* env = std::getenv("WLC_TX");
+  if (env) {
+    int t = std::atoi(env);
+    if (t >= 1 && (long long)M * t * nz * 16 <= 160 * 1024) return std::min(t, nxh);
+  }
+  for (int t : {4, 2, 1})
+    if ((long long)M * t * nz * 16 <= 104 * 1024 && t <= nxh) return t;
+  return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wlc_mde_last_error(void) { return g_last_error.c_str(); }
+
+int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
+  if (!out) return fail(WLC_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = validate(p);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    if (!g_tab_built) {
+      std::string err = wlc::build_orientation_tables(wlc::kMaxL, &g_tab);
+      if (!err.empty()) return fail(WLC_EINVAL, "orientation tables: " + err);
+      g_tab_built = true;
+    }
+  }
+  wlc_handle* h = new wlc_handle();
+  h->p = *p;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return fail(WLC_EHIP, "no HIP device");
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    if ((int)g_tab_uploaded.size() <= h->device) g_tab_uploaded.resize(h->device + 1, 0);
+    if (!g_tab_uploaded[h->device]) {
+      if (wlc::upload_orientation_tables(g_tab) != hipSuccess) {
+        delete h;
+        return fail(WLC_EHIP, "upload of orientation tables failed");
+      }
+      g_tab_uploaded[h->device] = 1;
+    }
+  }
+  h->N = (long long)p->nx * p->ny * p->nz;
+  h->nxh = p->nx / 2 + 1;
+  h->Nk = (long long)h->nxh * p->ny * p->nz;
+  h->Ns = p->Ns;
+  h->ds = 1.0 / p->Ns;  // initialize_mod.f90:140
+  int st = 0;
+  for (int b = 0; b < p->nblk; ++b) {
+    h->Lb[b] = p->Lmax[b];
+    h->Mb[b] = (p->Lmax[b] + 1) * (p->Lmax[b] + 1);
+    h->start[b] = st;
+    st += p->Nblk[b];
+    h->Mmax = std::max(h->Mmax, h->Mb[b]);
+    h->TX[b] = pick_tx(h->Mb[b], p->nz, h->nxh);
+    std::vector<double> cw;
+    contour_weights(p->Nblk[b], h->ds, &cw);
+    h->cw.push_back(cw);
+  }
+  h->off.resize(h->Ns + 2);
+  h->off[0] = 0;
+  for (int s = 0; s <= h->Ns; ++s) h->off[s + 1] = h->off[s] + (long long)h->Mb[block_of(h, s)] * h->N;
+  const int edges[3] = {p->nx, p->ny, p->nz};
+  for (int a = 0; a < 3; ++a) {
+    std::string err = wlc::make_axis_plan(edges[a], &h->plan[a]);
+    if (!err.empty()) { free_handle(h); return fail(WLC_EINVAL, err); }
+    if (h->plan[a].npass > 8) { free_handle(h); return fail(WLC_EINVAL, "too many FFT passes"); }
+  }
+#define ALLOC(ptr, bytes)                                                        \
+  do {                                                                           \
+    if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                      \
+      free_handle(h);                                                            \
+      return fail(WLC_ENOMEM, std::string("device allocation failed: ") + #ptr); \
+    }                                                                            \
+  } while (0)
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+    free_handle(h);
+    return fail(WLC_EHIP, "stream creation failed");
+  }
+  ALLOC(h->q, sizeof(double) * (size_t)h->off[h->Ns + 1]);
+  if (p->keep_qstar) ALLOC(h->qs, sizeof(double) * (size_t)h->off[h->Ns + 1]);
+  else ALLOC(h->qs, sizeof(double) * (size_t)4 * h->Mmax * h->N);
+  ALLOC(h->spec, sizeof(double2) * (size_t)h->Mmax * h->Nk);
+  ALLOC(h->w, sizeof(double) * (size_t)p->nblk * h->N);
+  ALLOC(h->rho, sizeof(double) * (size_t)p->nblk * h->N);
+  ALLOC(h->phi, sizeof(double) * (size_t)p->nblk * h->N);
+  ALLOC(h->partial, sizeof(double) * h->nparts);
+  ALLOC(h->scal, sizeof(double) * 4);
+  for (int a = 0; a < 3; ++a) {
+    ALLOC(h->tw[a], sizeof(double2) * (size_t)edges[a]);
+    if (hipMemcpy(h->tw[a], h->plan[a].tw.data(), sizeof(double2) * edges[a], hipMemcpyHostToDevice) != hipSuccess) {
+      free_handle(h);
+      return fail(WLC_EHIP, "twiddle upload failed");
+    }
+  }
+#undef ALLOC
+  wlc::Geom& g = h->geom;
+  g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nxh = h->nxh;
+  wlc::AxisDev* ax[3] = {&g.ax, &g.ay, &g.az};
+  for (int a = 0; a < 3; ++a) {
+    ax[a]->n = h->plan[a].n;
+    ax[a]->npass = h->plan[a].npass;
+    for (int i = 0; i < 8; ++i) ax[a]->radix[i] = i < h->plan[a].npass ? h->plan[a].radix[i] : 0;
+    ax[a]->tw = h->tw[a];
+  }
+  *out = h;
+  g_last_error.clear();
+  return WLC_OK;
+}
+
+void wlc_mde_destroy(wlc_handle* h) { free_handle(h); }
+
+int wlc_mde_set_box(wlc_handle* h, const double box[3]) {
+  if (!h || !box) return fail(WLC_EINVAL, "NULL argument");
+  for (int a = 0; a < 3; ++a)
+    if (!(box[a] > 0.0) || !std::isfinite(box[a])) return fail(WLC_EINVAL, "box edges must be positive");
+  for (int a = 0; a < 3; ++a) h->p.box[a] = box[a];
+  h->fwd_done = h->bwd_done = false;
+  return WLC_OK;
+}
+
+int wlc_mde_set_w(wlc_handle* h, const double* w_host) {
+  if (!h || !w_host) return fail(WLC_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(h->w, w_host, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->have_w = true;
+  h->fwd_done = h->bwd_done = false;
+  return WLC_OK;
+}
+
+int wlc_mde_set_w_device(wlc_handle* h, const double* w_dev) {
+  if (!h || !w_dev) return fail(WLC_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(h->w, w_dev, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyDeviceToDevice, h->st));
+  h->have_w = true;
+  h->fwd_done = h->bwd_done = false;
+  return WLC_OK;
+}
+
+int wlc_mde_forward(wlc_handle* h) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede forward");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(wlc::launch_set_delta(h->q, h->Mb[block_of(h, 0)], h->N, h->st));
+  for (int s = 1; s <= h->Ns; ++s) {
+    int rc = run_step(h, +1, s);
+    if (rc) return rc;
+  }
+  h->fwd_done = true;
+  h->bwd_done = false;
+  if (h->timing) return flush_timing(h);
+  return WLC_OK;
+}
+
+int wlc_mde_backward(wlc_handle* h) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede backward");
+  HIPCHK(hipSetDevice(h->device));
+  const int Ns = h->Ns;
+  HIPCHK(wlc::launch_fill(h->rho, 0.0, (long long)h->p.nblk * h->N, h->st));
+  HIPCHK(wlc::launch_set_delta(qs_slot(h, Ns), h->Mb[block_of(h, Ns)], h->N, h->st));
+  const bool dens = h->fwd_done;
+  if (dens) {
+    int rc = density_at(h, Ns);
+    if (rc) return rc;
+  }
+  for (int s = Ns - 1; s >= 0; --s) {
+    int rc = run_step(h, -1, s);
+    if (rc) return rc;
+    if (dens) {
+      rc = density_at(h, s);
+      if (rc) return rc;
+    }
+  }
+  h->bwd_done = dens;
+  if (h->timing) return flush_timing(h);
+  return WLC_OK;
+}
+
+static int density_common(wlc_handle* h, double* Q, double* totden) {
+  if (!h->fwd_done || !h->bwd_done) return fail(WLC_ESTATE, "density needs a forward and a backward sweep on the current w");
+  HIPCHK(hipSetDevice(h->device));
+  const double dV = (h->p.box[0] / h->p.nx) * (h->p.box[1] / h->p.ny) * (h->p.box[2] / h->p.nz);
+  const double V = h->p.box[0] * h->p.box[1] * h->p.box[2];
+  HIPCHK(wlc::launch_weighted_sum(h->geom, h->q + h->off[h->Ns], 1, dV, h->partial, h->nparts, h->scal, h->st));
+  HIPCHK(wlc::launch_weighted_sum(h->geom, h->rho, h->p.nblk, dV, h->partial, h->nparts, h->scal + 1, h->st));
+  HIPCHK(wlc::launch_phi(h->rho, h->p.nblk, h->N, V, h->scal + 1, h->phi, h->st));
+  double sc[2];
+  HIPCHK(hipMemcpyAsync(sc, h->scal, sizeof(sc), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  const double Qv = sc[0] / V;
+  if (!std::isfinite(Qv) || !std::isfinite(sc[1]) || sc[1] == 0.0) return fail(WLC_ENONFINITE, "Q or totden not finite");
+  if (Q) *Q = Qv;
+  if (totden) *totden = sc[1];
+  return WLC_OK;
+}
+
+int wlc_mde_density(wlc_handle* h, double* phi_host, double* Q, double* totden) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  int rc = density_common(h, Q, totden);
+  if (rc) return rc;
+  if (phi_host) {
+    HIPCHK(hipMemcpy(phi_host, h->phi, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyDeviceToHost));
+    for (long long e = 0; e < (long long)h->p.nblk * h->N; ++e)
+      if (!std::isfinite(phi_host[e])) return fail(WLC_ENONFINITE, "phi not finite");
+  }
+  return WLC_OK;
+}
+
+int wlc_mde_density_device(wlc_handle* h, double* phi_dev, double* Q, double* totden) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  int rc = density_common(h, Q, totden);
+  if (rc) return rc;
+  if (phi_dev) HIPCHK(hipMemcpyAsync(phi_dev, h->phi, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyDeviceToDevice, h->st));
+  return WLC_OK;
+}
+
+int wlc_mde_get_q(wlc_handle* h, int dir, int s, double* q_host) {
+  if (!h || !q_host) return fail(WLC_EINVAL, "NULL argument");
+  if (s < 0 || s > h->Ns || (dir != 1 && dir != -1)) return fail(WLC_EINVAL, "bad slice");
+  HIPCHK(hipSetDevice(h->device));
+  const long long M = h->Mb[block_of(h, s)];
+  const double* src;
+  if (dir > 0) {
+    if (!h->fwd_done) return fail(WLC_ESTATE, "no forward sweep");
+    src = h->q + h->off[s];
+  } else {
+    if (!h->p.keep_qstar && s != 0) return fail(WLC_ESTATE, "backward slices other than 0 need keep_qstar");
+    src = qs_slot(h, s);
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(q_host, src, sizeof(double) * (size_t)(M * h->N), hipMemcpyDeviceToHost));
+  return WLC_OK;
+}
+
+int wlc_mde_slice_M(const wlc_handle* h, int s) {
+  if (!h || s < 0 || s > h->Ns) return WLC_EINVAL;
+  return h->Mb[block_of(h, s)];
+}
+
+int wlc_mde_block_of(const wlc_handle* h, int s) {
+  if (!h || s < 0 || s > h->Ns) return WLC_EINVAL;
+  return block_of(h, s);
+}
+
+int wlc_mde_sync(wlc_handle* h) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  return WLC_OK;
+}
+
+void* wlc_mde_stream(wlc_handle* h) { return h ? (void*)h->st : nullptr; }
+
+int wlc_mde_set_timing(wlc_handle* h, int enable) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  int rc = flush_timing(h);
+  if (rc) return rc;
+  h->timing = enable != 0;
+  for (int k = 0; k < 4; ++k) { h->ms[k] = 0.0; h->cnt[k] = 0; }
+  return WLC_OK;
+}
+
+int wlc_mde_get_timing(wlc_handle* h, double* ms, long long* count, int nk) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  int rc = flush_timing(h);
+  if (rc) return rc;
+  for (int k = 0; k < nk && k < 4; ++k) {
+    if (ms) ms[k] = h->ms[k];
+    if (count) count[k] = h->cnt[k];
+  }
+  return WLC_OK;
+}
+
+int wlc_mde_event_record(wlc_handle* h, int slot) {
+  if (!h || slot < 0 || slot >= 8) return fail(WLC_EINVAL, "bad event slot");
+  if (!h->uev[slot]) HIPCHK(hipEventCreate(&h->uev[slot]));
+  HIPCHK(hipEventRecord(h->uev[slot], h->st));
+  return WLC_OK;
+}
+
+int wlc_mde_event_elapsed(wlc_handle* h, int a, int b, double* ms) {
+  if (!h || a < 0 || a >= 8 || b < 0 || b >= 8 || !h->uev[a] || !h->uev[b] || !ms)
+    return fail(WLC_EINVAL, "bad event slots");
+  HIPCHK(hipEventSynchronize(h->uev[b]));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, h->uev[a], h->uev[b]));
+  *ms = t;
+  return WLC_OK;
+}
+
+int wlc_test_solve(int L, int nk, const double* kvec_host, double cdelta, double sigma, double ds,
+                   double dslam, double* v_host) {
+  if (L < 0 || L > WLC_MAX_L || nk <= 0 || !kvec_host || !v_host) return fail(WLC_EINVAL, "bad arguments");
+  {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    if (!g_tab_built) {
+      std::string err = wlc::build_orientation_tables(wlc::kMaxL, &g_tab);
+      if (!err.empty()) return fail(WLC_EINVAL, "orientation tables: " + err);
+      g_tab_built = true;
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if ((int)g_tab_uploaded.size() <= dev) g_tab_uploaded.resize(dev + 1, 0);
+    if (!g_tab_uploaded[dev]) {
+      HIPCHK(wlc::upload_orientation_tables(g_tab));
+      g_tab_uploaded[dev] = 1;
+    }
+  }
+  const int M = (L + 1) * (L + 1);
+  double* dk = nullptr;
+  double2* dv = nullptr;
+  HIPCHK(hipMalloc((void**)&dk, sizeof(double) * 3 * (size_t)nk));
+  if (hipMalloc((void**)&dv, sizeof(double2) * (size_t)nk * M) != hipSuccess) {
+    (void)hipFree(dk);
+    return fail(WLC_ENOMEM, "test buffer allocation failed");
+  }
+  int rc = WLC_OK;
+  hipError_t e = hipMemcpy(dk, kvec_host, sizeof(double) * 3 * (size_t)nk, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dv, v_host, sizeof(double2) * (size_t)nk * M, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = wlc::launch_test_solve(L, nk, dk, cdelta, sigma, ds, dslam, dv, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(v_host, dv, sizeof(double2) * (size_t)nk * M, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = fail(WLC_EHIP, std::string("test solve: ") + hipGetErrorString(e));
+  (void)hipFree(dk);
+  (void)hipFree(dv);
+  return rc;
+}
+
+}  // extern "C"
