@@ -26,16 +26,18 @@ struct RhsArgs {
   double ds;
   const double* w;     // field of the target block [nz][ny][nx]
   int M;               // harmonics of the target slice
-  double2* spec;       // [M][nz][ny][nxh]
+  double2* spec;       // [M][ny][nxh][nz] (z fastest)
 };
 
 // Kernel B: z pass of the FFT, per-k solve of G(k) x = P(k), inverse z pass.
 struct SolveArgs {
   double2* spec;
-  int M, L, TX;        // harmonics, Lmax, kx columns per workgroup
+  int M, L, TX;        // harmonics, Lmax, kx columns per workgroup (TX * nz threads)
   double cdelta, sigma, ds, dslam;
   double kfac[3];      // 2 pi / box per axis
   int solve;           // 0: FFT round trip only (test hook)
+  int generic;         // 1: force the generic (any nz) kernel
+  int occ;             // nz = 64 kernel: target waves per SIMD (1 or 2)
 };
 
 // Kernel C: inverse y and x (C2R) passes, scaling by 1/N, store of q(s).
@@ -46,7 +48,7 @@ struct InvArgs {
   double* out;         // [M][nz][ny][nx]
 };
 
-hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, hipStream_t st);
+hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_t st);
 hipError_t launch_zsolve(const Geom& g, const SolveArgs& a, hipStream_t st);
 hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st);
 

@@ -91,6 +91,8 @@ struct wlc_handle {
   wlc::Geom geom{};
   hipStream_t st = nullptr;
   bool have_w = false, fwd_done = false, bwd_done = false;
+  int generic_zsolve = 0;
+  int zs_occ = 1;
   // timing
   bool timing = false;
   struct Rec { int k; hipEvent_t a, b; };
@@ -195,7 +197,7 @@ int run_step(wlc_handle* h, int dir, int s) {
   ra.spec = h->spec;
   {
     TimedLaunch t(h, 0);
-    HIPCHK(wlc::launch_rhs_fft(h->geom, ra, h->st));
+    HIPCHK(wlc::launch_rhs_fft(h->geom, ra, order, h->st));
   }
   wlc::SolveArgs sa{};
   sa.spec = h->spec;
@@ -209,6 +211,8 @@ int run_step(wlc_handle* h, int dir, int s) {
   const double tp = 2.0 * M_PI;
   for (int a = 0; a < 3; ++a) sa.kfac[a] = tp / h->p.box[a];
   sa.solve = 1;
+  sa.generic = h->generic_zsolve;
+  sa.occ = h->zs_occ;
   {
     TimedLaunch t(h, 1);
     HIPCHK(wlc::launch_zsolve(h->geom, sa, h->st));
@@ -293,15 +297,9 @@ int validate(const wlc_params* p) {
   return WLC_OK;
 }
 
-int pick_tx(int M, int nz, int nxh) {
-  const char* env = std::getenv("WLC_TX");
-  if (env) {
-    int t = std::atoi(env);
-    if (t >= 1 && (long long)M * t * nz * 16 <= 160 * 1024) return std::min(t, nxh);
-  }
-  for (int t : {4, 2, 1})
-    if ((long long)M * t * nz * 16 <= 104 * 1024 && t <= nxh) return t;
-  return 1;
+int pick_tx(int nz) {
+  // kernel B: one thread per k-point, TX kx columns of nz points per workgroup
+  return nz >= 256 ? 1 : 256 / nz;
 }
 
 }  // namespace
@@ -325,6 +323,8 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   }
   wlc_handle* h = new wlc_handle();
   h->p = *p;
+  if (const char* ge = std::getenv("WLC_GENERIC_ZSOLVE")) h->generic_zsolve = std::atoi(ge) != 0;
+  if (const char* oc = std::getenv("WLC_ZSOLVE_OCC")) h->zs_occ = std::atoi(oc) == 2 ? 2 : 1;
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
     return fail(WLC_EHIP, "no HIP device");
@@ -335,7 +335,7 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     if (!g_tab_uploaded[h->device]) {
       if (wlc::upload_orientation_tables(g_tab) != hipSuccess) {
         delete h;
-        return fail(WLC_EHIP, "upload of orientation tables failed");
+        return fail(WLC_EHIP, "compiled-in orientation tables differ from the host set-up (regenerate wlc_tables.inc)");
       }
       g_tab_uploaded[h->device] = 1;
     }
@@ -352,7 +352,7 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     h->start[b] = st;
     st += p->Nblk[b];
     h->Mmax = std::max(h->Mmax, h->Mb[b]);
-    h->TX[b] = pick_tx(h->Mb[b], p->nz, h->nxh);
+    h->TX[b] = pick_tx(p->nz);
     std::vector<double> cw;
     contour_weights(p->Nblk[b], h->ds, &cw);
     h->cw.push_back(cw);

@@ -1,0 +1,242 @@
+// Kernel B of the propagator step for one Lmax (this file is compiled once per L
+// with -DWLC_L=<L>): z pass of the forward FFT, k-aligned solve of G(k) q = P(k)
+// (replaces the per-k dense G^-1 apply of wormlike_MDE_2dcomp_mod.f90:395-409 with
+// the tables built by init_inverse_sparse_GAB, Gmatrix_2dcomp_mod.f90:222-650),
+// inverse z pass.
+//
+// One thread owns one k-point (kx, ky, kz) and holds all M harmonics of it in
+// registers.  The half spectrum is laid out [i][ky][kx][z] (z fastest), so a
+// column of one harmonic is contiguous.
+#include <hip/hip_runtime.h>
+
+#include "wlc_device.h"
+
+#ifndef WLC_L
+#error "compile with -DWLC_L=<Lmax>"
+#endif
+
+#define WLC_CAT2(a, b) a##b
+#define WLC_CAT(a, b) WLC_CAT2(a, b)
+
+namespace wlc {
+namespace WLC_CAT(per_l_, WLC_L) {
+
+namespace {
+constexpr int kL = WLC_L;
+constexpr int kM = (kL + 1) * (kL + 1);
+constexpr int kHB = 8;  // harmonics per LDS batch of the z transform
+}  // namespace
+
+template <int DIR>
+__device__ __forceinline__ void z_transform(double2 (&v)[kM], double2* lds, const AxisDev& az,
+                                            int TX, int lidx) {
+  const int nz = az.n;
+#pragma unroll
+  for (int b0 = 0; b0 < kM; b0 += kHB) {
+    const int nb = (kM - b0) < kHB ? (kM - b0) : kHB;
+#pragma unroll
+    for (int h = 0; h < kHB; ++h)
+      if (b0 + h < kM) lds[h * TX * nz + lidx] = v[b0 + h];
+    __syncthreads();
+    fft_axis<DIR, kHB>(lds, az, nb * TX, 1, nz);
+#pragma unroll
+    for (int h = 0; h < kHB; ++h)
+      if (b0 + h < kM) v[b0 + h] = lds[h * TX * nz + lidx];
+    __syncthreads();
+  }
+}
+
+// generic kernel B: any nz <= 256; TX columns of one ky row per workgroup, z transforms
+// through LDS eight harmonics at a time (block-wide Stockham passes)
+__global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
+  extern __shared__ double2 lds[];
+  const int nz = g.nz, ny = g.ny, nxh = g.nxh;
+  const int TX = a.TX;  // TX * nz == blockDim.x
+  const int t = threadIdx.x;
+  const int tx = t / nz, z = t - tx * nz;
+  const int kx = blockIdx.x * TX + tx, ky = blockIdx.y;
+  const bool valid = kx < nxh;
+  // spectrum layout [i][ky][kx][z]
+  const long long istride = (long long)ny * nxh * nz;
+  double2* p = a.spec + ((long long)ky * nxh + (valid ? kx : 0)) * nz + z;
+  double2 v[kM];
+#pragma unroll
+  for (int i = 0; i < kM; ++i) v[i] = valid ? p[i * istride] : make_double2(0.0, 0.0);
+  const int lidx = tx * nz + z;
+  z_transform<-1>(v, lds, g.az, TX, lidx);
+  if (a.solve && valid) {
+    const double kxv = signed_k(kx, g.nx) * a.kfac[0];
+    const double kyv = signed_k(ky, ny) * a.kfac[1];
+    const double kzv = signed_k(z, nz) * a.kfac[2];
+    Solver<kL>::solve(v, kxv, kyv, kzv, a.cdelta, a.sigma, a.ds, a.dslam);
+  }
+  z_transform<+1>(v, lds, g.az, TX, lidx);
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < kM; ++i) p[i * istride] = v[i];
+  }
+}
+
+// kernel B for nz = 64: one wave owns one (kx, ky) column, lane = z; the 64-point z
+// transform is an 8 x 8 four-step done wave-locally in LDS, eight harmonics per
+// batch (no workgroup barriers); each lane's step-1 twiddles stay in registers.
+// Forward: step 1 reads the column straight from global memory (next batch
+// prefetched), the result lands in v[] (lane = kz).  Inverse: v[] -> LDS -> step 1
+// -> step 2, whose outputs go straight to global memory.  LDS rows of 8 are padded
+// to 9 so the strided step-2 reads are bank-conflict free.
+__device__ __forceinline__ int pad8(int u) { return u + (u >> 3); }
+constexpr int kPadBatch = 72;  // padded units per harmonic (64 + 8)
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void z64_load_batch(double2 (&u)[8], const double2* __restrict__ col,
+                                               long long istride, int b0, int h, int j, bool ok) {
+  const bool in = ok && (b0 + h < kM);
+  const double2* src = col + (long long)(in ? b0 + h : 0) * istride + j;
+#pragma unroll
+  for (int n2 = 0; n2 < 8; ++n2) u[n2] = in ? src[8 * n2] : make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ void z64_forward(double2 (&v)[kM], double2* wl, int lane,
+                                            const double2 (&tw)[8], const double2* __restrict__ col,
+                                            long long istride, bool ok) {
+  const int h = lane >> 3, j = lane & 7;
+  double2 u[8], nxt[8];
+  z64_load_batch(nxt, col, istride, 0, h, j, ok);
+#pragma unroll
+  for (int b0 = 0; b0 < kM; b0 += kHB) {
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) u[n2] = nxt[n2];
+    if (b0 + kHB < kM) z64_load_batch(nxt, col, istride, b0 + kHB, h, j, ok);
+    Dft<8, -1>::run(u);
+#pragma unroll
+    for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<-1>(u[k2], tw[k2]);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
+    wave_sync();
+    // lane (h, k2 = j): DFT over the first index
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
+    Dft<8, -1>::run(u);
+    wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < kM) v[b0 + q] = wl[q * kPadBatch + pad8(lane)];
+    wave_sync();
+  }
+}
+
+__device__ __forceinline__ void z64_inverse(const double2 (&v)[kM], double2* wl, int lane,
+                                            const double2 (&tw)[8], double2* __restrict__ col,
+                                            long long istride, bool ok) {
+  const int h = lane >> 3, j = lane & 7;
+#pragma unroll
+  for (int b0 = 0; b0 < kM; b0 += kHB) {
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < kM) wl[q * kPadBatch + pad8(lane)] = v[b0 + q];
+    wave_sync();
+    double2 u[8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
+    Dft<8, +1>::run(u);
+#pragma unroll
+    for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<+1>(u[k2], tw[k2]);
+    wave_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
+    wave_sync();
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
+    Dft<8, +1>::run(u);
+    if (ok && b0 + h < kM) {
+      double2* dst = col + (long long)(b0 + h) * istride + j;
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) dst[8 * k1] = u[k1];
+    }
+    wave_sync();
+  }
+}
+
+template <int OCC>
+__global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
+  __shared__ double2 lds[4 * kHB * kPadBatch];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ny = g.ny, nxh = g.nxh;
+  const int col = blockIdx.x * 4 + wv;  // linear column index, kx fastest
+  const bool valid = col < nxh * ny;
+  const int ky = valid ? col / nxh : 0;
+  const int kx = valid ? col - ky * nxh : 0;
+  const long long istride = (long long)ny * nxh * 64;
+  double2* cp = a.spec + (long long)(valid ? col : 0) * 64;
+  // step-1 twiddles exp(-2 pi i j k2 / 64), j = lane & 7
+  double2 tw[8];
+  {
+    const int j = lane & 7;
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) tw[k2] = g.az.tw[j * k2];
+  }
+  double2* wl = lds + wv * kHB * kPadBatch;
+  double2 v[kM];
+  z64_forward(v, wl, lane, tw, cp, istride, valid);
+  if (a.solve && valid) {
+    const double kxv = signed_k(kx, g.nx) * a.kfac[0];
+    const double kyv = signed_k(ky, ny) * a.kfac[1];
+    const double kzv = signed_k(lane, 64) * a.kfac[2];
+    Solver<kL>::solve(v, kxv, kyv, kzv, a.cdelta, a.sigma, a.ds, a.dslam);
+  }
+  z64_inverse(v, wl, lane, tw, cp, istride, valid);
+}
+
+}  // namespace per_l_<L>
+
+using namespace WLC_CAT(per_l_, WLC_L);
+
+hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hipStream_t st) {
+  if (a.M != kM) return hipErrorInvalidValue;
+  if (g.nz == 64 && !a.generic) {
+    const int ncol = g.nxh * g.ny;
+    if (a.occ == 2) hipLaunchKernelGGL(k_zsolve64<2>, dim3((ncol + 3) / 4), dim3(256), 0, st, g, a);
+    else hipLaunchKernelGGL(k_zsolve64<1>, dim3((ncol + 3) / 4), dim3(256), 0, st, g, a);
+    return hipGetLastError();
+  }
+  if (a.TX * g.nz > 256 || a.TX < 1) return hipErrorInvalidValue;
+  const size_t lds = sizeof(double2) * (size_t)kHB * a.TX * g.nz;
+  hipError_t e = hipFuncSetAttribute((const void*)k_zsolve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.nxh + a.TX - 1) / a.TX, g.ny), block(a.TX * g.nz);
+  hipLaunchKernelGGL(k_zsolve, grid, block, lds, st, g, a);
+  return hipGetLastError();
+}
+
+namespace WLC_CAT(per_l_, WLC_L) {
+__global__ void k_test_solve(int nk, const double* kvec, double cdelta, double sigma, double ds,
+                             double dslam, double2* v) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nk) return;
+  double2 x[kM];
+#pragma unroll
+  for (int i = 0; i < kM; ++i) x[i] = v[(long long)p * kM + i];
+  Solver<kL>::solve(x, kvec[3 * p], kvec[3 * p + 1], kvec[3 * p + 2], cdelta, sigma, ds, dslam);
+#pragma unroll
+  for (int i = 0; i < kM; ++i) v[(long long)p * kM + i] = x[i];
+}
+
+}  // namespace per_l_<L>
+
+hipError_t WLC_CAT(test_solve_L, WLC_L)(int nk, const double* kvec, double cdelta, double sigma,
+                                        double ds, double dslam, double2* v, hipStream_t st) {
+  hipLaunchKernelGGL(k_test_solve, dim3((nk + 63) / 64), dim3(64), 0, st, nk, kvec, cdelta, sigma,
+                     ds, dslam, v);
+  return hipGetLastError();
+}
+
+
+}  // namespace wlc
