@@ -1,0 +1,183 @@
+"""Benchmark: MDE contour-steps/s of the wormlike-chain propagator solve on one GPU.
+
+Workload (the kernel benchmark configuration of the reference's MDE path): 64^3
+periodic grid, box 5.0, three blocks of equal fraction (Ns = 200 -> 67/67/66 contour
+steps), Lmax = 6 (49 harmonics) and lambda = 1.0 in every block; w_A,B,C are i.i.d.
+N(0,1) from seed 1, smoothed by a periodic Gaussian filter of width 2 cells and
+scaled to standard deviation 5; q(0) = delta_{l0}.  One bench step is one forward
+sweep, one backward sweep (with the fused contour quadrature) and the density
+reduction: 2 * Ns = 400 contour steps.  Inputs are resident in HBM before the timed
+region starts.
+
+Prints one JSON line; see the repository's design notes for the field definitions.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "wormlike-chain-triblock_amd"))
+
+METRIC = "MDE contour-steps/s at 64^3 Lmax=6"
+PEAK_HBM_GBS = 8000.0
+KERNELS = ["rhs_fft_xy", "zsolve", "ifft_xy", "density_acc"]
+
+
+def workload_w(shape=(64, 64, 64), seed=1, sigma=2.0, std=5.0, nblk=3):
+    """Seeded block fields: N(0,1), periodic Gaussian filter exp(-sigma^2 |2 pi m/n|^2 / 2)
+    in cells, then each block scaled to the given standard deviation."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((nblk,) + tuple(shape))
+    nz, ny, nx = shape
+    kz = 2 * np.pi * np.fft.fftfreq(nz)[:, None, None]
+    ky = 2 * np.pi * np.fft.fftfreq(ny)[None, :, None]
+    kx = 2 * np.pi * np.fft.rfftfreq(nx)[None, None, :]
+    filt = np.exp(-0.5 * sigma ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
+    for b in range(nblk):
+        f = np.fft.irfftn(np.fft.rfftn(w[b]) * filt, s=shape)
+        f -= f.mean()
+        w[b] = f * (std / f.std())
+    return w
+
+
+def algorithmic_bytes(M, N, Nk):
+    """FP64 bytes each kernel must move per launch (BDF3 step); the step total is
+    the 4 M N 8 + 8 N + 64 M Nk figure of the field traffic."""
+    return {
+        "rhs_fft_xy": 3 * M * N * 8 + N * 8 + M * Nk * 16,
+        "zsolve": 2 * M * Nk * 16,
+        "ifft_xy": M * Nk * 16 + M * N * 8,
+        "density_acc": 2 * M * N * 8 + 2 * N * 8,
+        "step": 4 * M * N * 8 + 8 * N + 64 * M * Nk,
+    }
+
+
+def read_traffic(path):
+    """Per-launch HBM bytes per kernel from a committed rocprofv3 summary: lines
+    'traffic <kernel> <bytes>'."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 3 and p[0] == "traffic":
+                out[p[1]] = float(p[2])
+    return out
+
+
+def cpu_baseline(w, target_s=15.0):
+    """The oracle (C restatement of the reference's per-step dataflow: BDF3 RHS,
+    per-harmonic R2C FFT, dense G^-1 apply via its lower triangle, C2R FFT) timed on
+    this host on a bounded sample of the same workload: BDF3 steps of block A."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle  # noqa: E402  (checker / CPU baseline only)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc = oracle.Oracle((64, 64, 64), (5.0, 5.0, 5.0), (67, 67, 66), (6, 6, 6), (1.0, 1.0, 1.0),
+                        nthreads=threads)
+    t0 = time.time()
+    first = orc.time_bdf3_steps(w[0], 1)  # builds the G^-1 table (excluded) + 1 step
+    build_s = time.time() - t0 - first
+    n = max(2, min(64, int(target_s / max(first, 1e-3))))
+    dt = orc.time_bdf3_steps(w[0], n)
+    orc.close()
+    return {"value": n / dt, "unit": "contour-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} BDF3 contour steps of the 64^3 Lmax=6 workload (block A field), "
+                      f"dense G^-1 table build of {build_s:.1f} s excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed sweeps (each 2*Ns contour steps)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--traffic-file", default=os.path.join(HERE, "profiles", "traffic_64cube_L6.txt"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: the propagator solve runs on a single device "
+              f"(multi-GPU sharding is a design note, not built); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import wlc_mde as W
+
+    shape = (64, 64, 64)
+    Ns = 200
+    Nblk = W.block_lengths(Ns, [1 / 3, 1 / 3, 1 / 3])
+    L, lam, box = 6, 1.0, 5.0
+    M = (L + 1) ** 2
+    N = 64 ** 3
+    Nk = 64 * 64 * 33
+    w = workload_w(shape)
+    g = W.WormlikeMDE(shape, (box,) * 3, Nblk, (L,) * 3, (lam,) * 3)
+    g.set_w(w)  # host -> HBM before the timed region
+
+    for _ in range(args.warmup):
+        g.MDE_q()
+        g.MDE_qstar()
+        g.density_device()
+    g.sync()
+    t0 = time.perf_counter()
+    g.event_record(0)
+    Q = td = None
+    for _ in range(args.steps):
+        g.MDE_q()
+        g.MDE_qstar()
+        Q, td = g.density_device()
+    g.event_record(1)
+    ms = g.event_elapsed(0, 1)
+    g.sync()
+    wall = time.perf_counter() - t0
+    if not (np.isfinite(Q) and np.isfinite(td)):
+        raise RuntimeError("non-finite Q/totden")
+    contour_steps = 2 * Ns * args.steps
+    value = contour_steps / (ms / 1e3)
+
+    # per-kernel average launch durations (HIP events on the handle's stream)
+    g.set_timing(True)
+    g.MDE_q()
+    g.MDE_qstar()
+    kms, kcnt = g.get_timing()
+    g.set_timing(False)
+    avg_us = {k: (1e3 * m / c if c else 0.0) for k, m, c in zip(KERNELS, kms, kcnt)}
+    ab = algorithmic_bytes(M, N, Nk)
+    dom = max(KERNELS, key=lambda k: kms[KERNELS.index(k)])
+    achieved = ab[dom] / (avg_us[dom] * 1e-6) / 1e9
+    traffic = read_traffic(args.traffic_file).get(dom)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": dom,
+            "algorithmic_bytes": ab[dom], "avg_launch_us": round(avg_us[dom], 2)}
+    step_s = ms / 1e3 / contour_steps
+    step_roof = {"algorithmic_bytes": ab["step"], "achieved": round(ab["step"] / step_s / 1e9, 1),
+                 "frac": round(ab["step"] / step_s / 1e9 / PEAK_HBM_GBS, 4)}
+    g.close()
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "contour-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "MDE forward+backward sweep, 64^3, box 5.0, Lmax=6 (49 harmonics), "
+                               "Ns=200 (67/67/66), lambda=1.0, smoothed N(0,1) w (seed 1, sigma 2 "
+                               "cells, std 5); one step = one sweep of 400 contour steps",
+                   "grid": [64, 64, 64], "Lmax": L, "Ns": Ns, "blocks": Nblk,
+                   "parallelism": "1 GPU"},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "kernels_avg_us": {k: round(v, 2) for k, v in avg_us.items()},
+        "wall_s": round(wall, 3),
+        "Q": Q,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

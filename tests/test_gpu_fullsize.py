@@ -1,0 +1,105 @@
+"""Full-size GPU checks.
+
+- 32^3, Lmax 4, Ns 100 ABC chain (the reference-style plain-SCFT configuration):
+  element-wise parity with the oracle.
+- 64^3, Lmax 6, Ns 200 (the benchmark sweep): size-independent properties that the
+  oracle cannot check quickly at this size -- w = 0 gives Q = 1 and phi = f; equal
+  block fields make the chain reversible, q*(0) = P q(Ns) with P = diag((-1)^l);
+  runs are bit-reproducible; an even-cell shift of w shifts phi and keeps Q.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def config1_w():
+    rng = np.random.default_rng(0)
+    shape = (32, 32, 32)
+    w = 0.1 * rng.standard_normal((3,) + shape)
+    z = np.arange(32) * (4.0 / 32)
+    c = 2.0 * np.cos(2 * np.pi * z / 4.0)[:, None, None]
+    w[0] += c
+    w[2] -= c
+    return w
+
+
+def test_config1_matches_oracle(wlc, oracle):
+    shape, box = (32, 32, 32), (4.0, 4.0, 4.0)
+    Nblk = wlc.block_lengths(100, [1 / 3] * 3)
+    L, lam = (4, 4, 4), (1.0, 1.0, 1.0)
+    w = config1_w()
+    print("config1: oracle sweep 32^3 L4 Ns100", flush=True)
+    orc = oracle.Oracle(shape, box, Nblk, L, lam)
+    q = orc.forward(w)
+    qs = orc.backward(w)
+    phi, Q, td = orc.density(q, qs)
+    chi = np.full((3, 3), 20.0)
+    F = orc.free_energy(w, phi, Q, chi)
+    g = wlc.WormlikeMDE(shape, box, Nblk, L, lam)
+    g.set_w(w)
+    g.MDE_q()
+    g.MDE_qstar()
+    gphi, gQ, gtd = g.density()
+    gF = orc.free_energy(w, gphi, gQ, chi)
+    assert abs(gQ - Q) / Q < TOL
+    assert abs(gtd - td) / td < TOL
+    assert _rel(gphi, phi) < TOL
+    assert abs(gF - F) < 1e-9                       # kT per chain
+    for s in (1, 33, 34, 66, 67, 100):
+        assert _rel(g.get_q(+1, s), orc.slice(q, s)) < TOL
+    assert _rel(g.get_q(-1, 0), orc.slice(qs, 0)) < TOL
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def c2(wlc):
+    shape = (64, 64, 64)
+    g = wlc.WormlikeMDE(shape, (5.0, 5.0, 5.0), wlc.block_lengths(200, [1 / 3] * 3), (6, 6, 6),
+                        (1.0, 1.0, 1.0))
+    yield g
+    g.close()
+
+
+def test_config2_free_chain(c2):
+    c2.set_w(np.zeros((3, 64, 64, 64)))
+    c2.MDE_q()
+    c2.MDE_qstar()
+    phi, Q, td = c2.density()
+    assert abs(Q - 1.0) < 1e-12
+    f = np.array([67, 67, 66]) / 200
+    for b in range(3):
+        assert np.abs(phi[b] - f[b]).max() < 1e-11
+
+
+def test_config2_reversal_determinism_and_shift(c2):
+    rng = np.random.default_rng(1)
+    w1 = rng.standard_normal((64, 64, 64))
+    w = np.stack([w1, w1, w1])
+    c2.set_w(w)
+    c2.MDE_q()
+    c2.MDE_qstar()
+    phi, Q, td = c2.density()
+    qN = c2.get_q(+1, 200)
+    qs0 = c2.get_q(-1, 0)
+    par = np.array([(-1) ** l for l in range(7) for m in range(-l, l + 1)])[:, None, None, None]
+    assert _rel(qs0, par * qN) < 1e-11
+    # bit-reproducible
+    c2.MDE_q()
+    c2.MDE_qstar()
+    phi2, Q2, _ = c2.density()
+    assert Q2 == Q
+    assert np.array_equal(phi2, phi)
+    # even-cell shift
+    ws = np.roll(w, (2, -4, 6), axis=(1, 2, 3))
+    c2.set_w(ws)
+    c2.MDE_q()
+    c2.MDE_qstar()
+    phis, Qs, _ = c2.density()
+    assert abs(Qs - Q) / Q < 1e-12
+    assert _rel(phis, np.roll(phi, (2, -4, 6), axis=(1, 2, 3))) < 1e-11

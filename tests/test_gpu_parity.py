@@ -1,0 +1,131 @@
+"""GPU parity of the HIP propagator solve against the CPU oracle (through the C ABI).
+
+Tolerance (FP64 path): 1e-10 relative in phi, Q and every propagator slice, the
+parity bar stated for the FP64 GPU path.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("L", list(range(9)))
+def test_k_aligned_solve_matches_dense_inverse(wlc, oracle, L):
+    rng = np.random.default_rng(100 + L)
+    M = (L + 1) ** 2
+    k = rng.standard_normal((64, 3)) * 25
+    k[0] = 0.0                       # k = 0: diagonal G
+    k[1, :2] = 0.0                   # k along z: no rotation needed
+    k[2, 2] = 0.0                    # k in the xy plane
+    k[3] = [40.0, 0.0, 0.0]
+    v = rng.standard_normal((64, M)) + 1j * rng.standard_normal((64, M))
+    for cd, sigma in [(1.0, 1.0), (1.5, -1.0), (11 / 6, 1.0), (11 / 6, -1.0)]:
+        got = wlc.test_solve(L, k, cd, sigma, 0.004, 0.004 * 0.9, v)
+        for p in range(64):
+            ref = oracle.ginv(L, cd, sigma, 0.004, 0.9, k[p]) @ v[p]
+            assert _rel(got[p], ref) < 1e-13
+
+
+CASES = [
+    # (shape (nz, ny, nx), box, Nblk, Lmax, lambda, force generic z kernel)
+    ((4, 6, 8), (4.0, 2.5, 3.0), (4, 5, 3), (2, 3, 1), (1.0, 0.5, 2.0), 0),
+    ((6, 12, 6), (2.0, 3.0, 2.5), (3, 5, 4), (1, 3, 2), (2.0, 0.3, 1.0), 0),
+    ((64, 6, 8), (3.0, 2.0, 6.0), (5, 4, 6), (3, 2, 4), (0.7, 1.3, 0.4), 0),
+    ((64, 6, 8), (3.0, 2.0, 6.0), (5, 4, 6), (3, 2, 4), (0.7, 1.3, 0.4), 1),
+    ((8, 8, 8), (4.0, 4.0, 4.0), (9,), (4,), (0.7,), 0),
+    ((24, 4, 6), (2.0, 2.0, 5.0), (6, 7), (0, 2), (1.0, 0.0), 0),
+    ((2, 2, 2), (1.0, 1.0, 1.0), (2, 1, 2), (8, 5, 6), (0.5, 1.0, 3.0), 0),
+    ((16, 4, 4), (2.0, 2.0, 2.0), (3, 3, 3), (6, 6, 6), (1.0, 1.0, 1.0), 0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[2]}-L{c[3]}-g{c[5]}" for c in CASES])
+def test_sweep_matches_oracle(wlc, oracle, case, monkeypatch):
+    shape, box, Nblk, L, lam, generic = case
+    monkeypatch.setenv("WLC_GENERIC_ZSOLVE", str(generic))
+    rng = np.random.default_rng(sum(shape) + len(Nblk))
+    w = 1.5 * rng.standard_normal((len(Nblk),) + shape)
+    orc = oracle.Oracle(shape, box, Nblk, L, lam)
+    q = orc.forward(w)
+    qs = orc.backward(w)
+    phi, Q, td = orc.density(q, qs)
+    g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, keep_qstar=True)
+    g.set_w(w)
+    g.MDE_q()
+    g.MDE_qstar()
+    gphi, gQ, gtd = g.density()
+    for s in range(orc.Ns + 1):
+        assert _rel(g.get_q(+1, s), orc.slice(q, s)) < TOL, f"forward slice {s}"
+        assert _rel(g.get_q(-1, s), orc.slice(qs, s)) < TOL, f"backward slice {s}"
+    assert abs(gQ - Q) / abs(Q) < TOL
+    assert abs(gtd - td) / abs(td) < TOL
+    assert _rel(gphi, phi) < TOL
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["ab_even", "abc_odd_radix3", "homopolymer"])
+def test_sweep_matches_golden_fixture(wlc, name):
+    f = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    g = wlc.WormlikeMDE(tuple(f["shape"]), tuple(f["box"]), tuple(f["Nblk"]), tuple(f["L"]),
+                        tuple(f["lam"]), keep_qstar=True)
+    g.set_w(f["w"])
+    g.MDE_q()
+    g.MDE_qstar()
+    phi, Q, td = g.density()
+    Ns = int(sum(f["Nblk"]))
+    assert _rel(g.get_q(+1, Ns), f["q_end"]) < TOL
+    assert _rel(g.get_q(+1, int(f["s_mid"])), f["q_mid"]) < TOL
+    assert _rel(g.get_q(-1, 0), f["qs_0"]) < TOL
+    assert _rel(phi, f["phi"]) < TOL
+    assert abs(Q - float(f["Q"])) / float(f["Q"]) < TOL
+    g.close()
+
+
+def test_state_errors(wlc):
+    g = wlc.WormlikeMDE((4, 4, 4), (2.0, 2.0, 2.0), (3, 3), (2, 2), (1.0, 1.0))
+    with pytest.raises(wlc.WlcError) as e:
+        g.MDE_q()
+    assert e.value.code == -4
+    g.set_w(np.zeros((2, 4, 4, 4)))
+    with pytest.raises(wlc.WlcError) as e:
+        g.density()
+    assert e.value.code == -4
+    g.MDE_q()
+    g.MDE_qstar()
+    g.density()
+    with pytest.raises(wlc.WlcError) as e:
+        g.get_q(-1, 3)               # backward slices need keep_qstar
+    assert e.value.code == -4
+    g.get_q(-1, 0)
+    with pytest.raises(ValueError):
+        g.set_w(np.zeros((3, 4, 4, 4)))
+    g.set_w(np.zeros((2, 4, 4, 4)))  # a new w invalidates the sweeps
+    with pytest.raises(wlc.WlcError):
+        g.density()
+    g.close()
+
+
+def test_set_box_changes_the_solution(wlc, oracle):
+    shape, Nblk, L, lam = (8, 8, 8), (4, 4), (3, 3), (1.0, 1.0)
+    rng = np.random.default_rng(9)
+    w = rng.standard_normal((2,) + shape)
+    g = wlc.WormlikeMDE(shape, (3.0, 3.0, 3.0), Nblk, L, lam)
+    g.set_w(w)
+    g.set_box((4.0, 3.5, 5.0))
+    g.set_w(w)
+    g.MDE_q()
+    g.MDE_qstar()
+    phi, Q, _ = g.density()
+    orc = oracle.Oracle(shape, (4.0, 3.5, 5.0), Nblk, L, lam)
+    ophi, oQ, _ = orc.density(orc.forward(w), orc.backward(w))
+    assert abs(Q - oQ) / oQ < TOL
+    assert _rel(phi, ophi) < TOL
+    g.close()
