@@ -1,0 +1,48 @@
+"""Build the Fortran binding (module wormlike_MDE) and the simple-mixing SCFT driver
+with ROCm flang, linked against libwlc_mde.so (rpath to the package directory)."""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+BUILD = os.path.join(PKG, "_build", "fortran")
+EXE = os.path.join(HERE, "scft_simple_mixing")
+
+
+def find_flang():
+    for c in ("amdflang", "flang", "/opt/rocm/bin/amdflang", "/opt/rocm/lib/llvm/bin/flang"):
+        p = shutil.which(c) or (c if os.path.exists(c) else None)
+        if p:
+            return p
+    return None
+
+
+def build():
+    fc = find_flang()
+    if fc is None:
+        print("build_fortran: no Fortran compiler found; skipping the Fortran binding")
+        return None
+    os.makedirs(BUILD, exist_ok=True)
+    mod_obj = os.path.join(BUILD, "wormlike_mde_gpu.o")
+    drv_obj = os.path.join(BUILD, "scft_simple_mixing.o")
+    tmp = EXE + ".tmp"
+    steps = [
+        [fc, "-O2", "-module-dir", BUILD, "-c", os.path.join(HERE, "wormlike_mde_gpu.f90"), "-o", mod_obj],
+        [fc, "-O2", "-module-dir", BUILD, "-I", BUILD, "-c", os.path.join(HERE, "scft_simple_mixing.f90"),
+         "-o", drv_obj],
+        [fc, "-o", tmp, mod_obj, drv_obj, f"-L{PKG}", "-lwlc_mde", "-Wl,-rpath,$ORIGIN/.."],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed ({' '.join(cmd)}):\n{r.stderr[-4000:]}")
+    os.replace(tmp, EXE)
+    print(f"built {EXE}")
+    return EXE
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

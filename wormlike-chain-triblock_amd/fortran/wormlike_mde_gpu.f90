@@ -1,0 +1,229 @@
+! Fortran drop-in for the reference's MDE module on MI355X.
+!
+! The reference driver calls, once per SCFT iteration (SCFT_control.f90:243-257),
+!     call MDE_q(.true.); call MDE_qstar(.true.); call density; call free_energy
+! on procedures of `module wormlike_MDE` (wormlike_MDE_2dcomp_mod.f90:227-244,
+! 961-973) and `density` / `free_energy` (SCFT_Density_FE.f90:3, 273), all reading and
+! writing module globals.  This file keeps those names and that calling style and
+! forwards the work through ISO_C_BINDING to libwlc_mde.so (include/wlc_mde.h).
+!
+! module wlc_mde_c    : bind(C) interfaces of the C ABI
+! module wormlike_MDE : the globals a driver sets (grid, box, blocks, fields) and
+!                       MDE_setup, MDE_q, MDE_qstar, density, free_energy, MDE_release
+module wlc_mde_c
+  use iso_c_binding
+  implicit none
+
+  type, bind(C) :: wlc_params
+     integer(c_int) :: nx, ny, nz
+     real(c_double) :: box(3)
+     integer(c_int) :: nblk, Ns
+     integer(c_int) :: ns_blk(3), lmax(3)   ! C: Nblk, Lmax
+     real(c_double) :: lambda(3)
+     integer(c_int) :: keep_qstar
+     integer(c_int) :: reserved(7)
+  end type wlc_params
+
+  interface
+     integer(c_int) function wlc_mde_create(p, h) bind(C, name="wlc_mde_create")
+       import :: wlc_params, c_int, c_ptr
+       type(wlc_params), intent(in) :: p
+       type(c_ptr), intent(out) :: h
+     end function wlc_mde_create
+
+     integer(c_int) function wlc_mde_set_box(h, box) bind(C, name="wlc_mde_set_box")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: box(3)
+     end function wlc_mde_set_box
+
+     integer(c_int) function wlc_mde_set_w(h, w) bind(C, name="wlc_mde_set_w")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: w(*)
+     end function wlc_mde_set_w
+
+     integer(c_int) function wlc_mde_forward(h) bind(C, name="wlc_mde_forward")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function wlc_mde_forward
+
+     integer(c_int) function wlc_mde_backward(h) bind(C, name="wlc_mde_backward")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function wlc_mde_backward
+
+     integer(c_int) function wlc_mde_density(h, phi, Q, totden) bind(C, name="wlc_mde_density")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: phi(*)
+       real(c_double) :: Q, totden
+     end function wlc_mde_density
+
+     subroutine wlc_mde_destroy(h) bind(C, name="wlc_mde_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end subroutine wlc_mde_destroy
+
+     type(c_ptr) function wlc_mde_last_error() bind(C, name="wlc_mde_last_error")
+       import :: c_ptr
+     end function wlc_mde_last_error
+
+     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+       import :: c_ptr, c_size_t
+       type(c_ptr), value :: s
+     end function c_strlen
+  end interface
+
+contains
+
+  subroutine wlc_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n, i
+    character(len=512) :: text
+    if (rc == 0) return
+    p = wlc_mde_last_error()
+    n = int(c_strlen(p))
+    text = ''
+    if (n > 0) then
+       call c_f_pointer(p, msg, [n])
+       do i = 1, min(n, 512)
+          text(i:i) = msg(i)
+       end do
+    end if
+    write(*,'(a,a,a,i0,a,a)') 'wormlike_MDE: ', where, ' failed, status ', rc, ': ', trim(text)
+    error stop 1
+  end subroutine wlc_check
+
+end module wlc_mde_c
+
+
+module wormlike_MDE
+  use iso_c_binding
+  use wlc_mde_c
+  implicit none
+
+  ! ---- globals the driver sets (the reference keeps these in global_para) ----
+  integer :: SIDEx = 0, SIDEy = 0, SIDEz = 0          ! grid
+  real(8) :: box_x = 0d0, box_y = 0d0, box_z = 0d0    ! box edges (SIDE*d)
+  integer :: n_blk = 3                                ! blocks (2: AB, 3: ABC)
+  integer :: Nmax = 0                                 ! contour steps, ds = 1/Nmax
+  integer :: Ns_blk(3) = 0                            ! NA, NB, NC
+  integer :: L_bar_blk(3) = 0                         ! Lmax per block
+  real(8) :: lambda_blk(3) = 1d0                      ! Loa/(2 kappa) per block
+  real(8), allocatable :: w_blk(:,:)                  ! (0:LOCAL_SIZE-1, 1:n_blk), x fastest
+  ! ---- outputs (density / free_energy) ----
+  real(8), allocatable :: R_blk(:,:)                  ! phi per block, same layout as w_blk
+  real(8) :: pff_global = 0d0                         ! ln Q   (SCFT_Density_FE.f90:91)
+  real(8) :: totden = 0d0
+  real(8) :: FE_global = 0d0                          ! F per chain (SCFT_Density_FE.f90:333)
+
+  type(c_ptr), private, save :: handle = c_null_ptr
+
+contains
+
+  ! create the GPU solver from the globals (replaces the derived-parameter part of
+  ! initialize, init_arrays_2decomp allocations and init_inverse_sparse_GAB)
+  subroutine MDE_setup()
+    type(wlc_params) :: p
+    integer :: b
+    if (c_associated(handle)) call MDE_release()
+    p%nx = SIDEx; p%ny = SIDEy; p%nz = SIDEz
+    p%box = [box_x, box_y, box_z]
+    p%nblk = n_blk
+    p%Ns = Nmax
+    p%ns_blk = 0; p%lmax = 0; p%lambda = 0d0
+    do b = 1, n_blk
+       p%ns_blk(b) = Ns_blk(b)
+       p%lmax(b) = L_bar_blk(b)
+       p%lambda(b) = lambda_blk(b)
+    end do
+    p%keep_qstar = 0
+    p%reserved = 0
+    call wlc_check(wlc_mde_create(p, handle), 'wlc_mde_create')
+    if (.not. allocated(w_blk)) allocate(w_blk(0:SIDEx*SIDEy*SIDEz-1, n_blk))
+    if (.not. allocated(R_blk)) allocate(R_blk(0:SIDEx*SIDEy*SIDEz-1, n_blk))
+  end subroutine MDE_setup
+
+  ! box change (the reference rebuilds G^-1 tables: init_inverse_sparse_GAB)
+  subroutine MDE_set_box(bx, by, bz)
+    real(8), intent(in) :: bx, by, bz
+    box_x = bx; box_y = by; box_z = bz
+    call wlc_check(wlc_mde_set_box(handle, [bx, by, bz]), 'wlc_mde_set_box')
+  end subroutine MDE_set_box
+
+  ! forward propagator (wormlike_MDE_2dcomp_mod.f90:231); uploads the current fields
+  subroutine MDE_q(R2C_FFT)
+    logical, optional :: R2C_FFT
+    call wlc_check(wlc_mde_set_w(handle, w_blk), 'wlc_mde_set_w')
+    call wlc_check(wlc_mde_forward(handle), 'wlc_mde_forward')
+  end subroutine MDE_q
+
+  ! backward propagator (wormlike_MDE_2dcomp_mod.f90:961)
+  subroutine MDE_qstar(R2C_FFT)
+    logical, optional :: R2C_FFT
+    call wlc_check(wlc_mde_backward(handle), 'wlc_mde_backward')
+  end subroutine MDE_qstar
+
+  ! Q, totden and phi per block (SCFT_Density_FE.f90:3-159)
+  subroutine density()
+    real(c_double) :: Q, td
+    call wlc_check(wlc_mde_density(handle, R_blk, Q, td), 'wlc_mde_density')
+    pff_global = log(Q)
+    totden = td
+  end subroutine density
+
+  ! F = -ln Q + (1/V) int [ sum_{a<b} chi_ab phi_a phi_b - sum_a w_a phi_a - xi (1 - sum_a phi_a) ],
+  ! xi = mean of the block fields; for two blocks this is SCFT_Density_FE.f90:296-298.
+  subroutine free_energy(chi)
+    real(8), intent(in) :: chi(3, 3)
+    real(8), allocatable :: ar(:)
+    integer :: r, a, b, n
+    real(8) :: xi, sphi, e
+    n = SIDEx * SIDEy * SIDEz
+    allocate(ar(0:n-1))
+    do r = 0, n - 1
+       xi = sum(w_blk(r, 1:n_blk)) / n_blk
+       sphi = sum(R_blk(r, 1:n_blk))
+       e = 0d0
+       do a = 1, n_blk
+          do b = a + 1, n_blk
+             e = e + chi(a, b) * R_blk(r, a) * R_blk(r, b)
+          end do
+          e = e - w_blk(r, a) * R_blk(r, a)
+       end do
+       ar(r) = e - xi * (1d0 - sphi)
+    end do
+    FE_global = -pff_global + periodic_simpson(ar) / (box_x * box_y * box_z)
+    deallocate(ar)
+  end subroutine free_energy
+
+  ! periodic Simpson volume integral, weights 2/3 (even index) and 4/3 (odd index)
+  ! per axis (utility_mod.f90:238-298)
+  real(8) function periodic_simpson(f) result(s)
+    real(8), intent(in) :: f(0:)
+    integer :: x, y, z
+    real(8) :: cx, cy, cz
+    s = 0d0
+    do z = 0, SIDEz - 1
+       cz = merge(4d0/3d0, 2d0/3d0, mod(z, 2) == 1)
+       do y = 0, SIDEy - 1
+          cy = merge(4d0/3d0, 2d0/3d0, mod(y, 2) == 1)
+          do x = 0, SIDEx - 1
+             cx = merge(4d0/3d0, 2d0/3d0, mod(x, 2) == 1)
+             s = s + cx * cy * cz * f(x + SIDEx * (y + SIDEy * z))
+          end do
+       end do
+    end do
+    s = s * (box_x / SIDEx) * (box_y / SIDEy) * (box_z / SIDEz)
+  end function periodic_simpson
+
+  subroutine MDE_release()
+    if (c_associated(handle)) call wlc_mde_destroy(handle)
+    handle = c_null_ptr
+  end subroutine MDE_release
+
+end module wormlike_MDE
