@@ -12,8 +12,20 @@ struct AxisDev {
   const double2* tw;  // exp(-2 pi i t / n), t < n
 };
 
+// Half spectrum layout: [i][ky][kx / 4][z][kx % 4] (complex FP64).  Plane kernels then
+// move whole 64-byte runs of 4 kx, and the four waves of a kernel-B workgroup (one per
+// kx of a tile) read interleaved pieces of the same cache lines.
+constexpr int kKxTile = 4;
+__host__ __device__ inline int kx_tiles(int nxh) { return (nxh + kKxTile - 1) / kKxTile; }
+__host__ __device__ inline long long spec_index(int i, int ky, int kx, int z, int ny, int nt, int nz) {
+  return ((((long long)i * ny + ky) * nt + (kx >> 2)) * nz + z) * kKxTile + (kx & 3);
+}
+
 struct Geom {
   int nx, ny, nz, nxh;
+  int nt;          // kx tiles
+  long long zs;    // real-space z-plane stride (doubles): nx*ny + pad
+  long long hs;    // real-space field / harmonic stride (doubles): nz*zs + pad
   AxisDev ax, ay, az;
 };
 
@@ -26,7 +38,8 @@ struct RhsArgs {
   double ds;
   const double* w;     // field of the target block [nz][ny][nx]
   int M;               // harmonics of the target slice
-  double2* spec;       // [M][ny][nxh][nz] (z fastest)
+  int hpw;             // harmonics per workgroup (the w plane is read once per group)
+  double2* spec;       // half spectrum, layout of spec_index
 };
 
 // Kernel B: z pass of the FFT, per-k solve of G(k) x = P(k), inverse z pass.
@@ -53,17 +66,17 @@ hipError_t launch_zsolve(const Geom& g, const SolveArgs& a, hipStream_t st);
 hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st);
 
 // small elementwise/reduction kernels
-hipError_t launch_set_delta(double* slice, int M, long long N, hipStream_t st);
+hipError_t launch_set_delta(const Geom& g, double* slice, int M, hipStream_t st);
 hipError_t launch_fill(double* dst, double v, long long n, hipStream_t st);
 // rho_t += wgt_t * sum_{i < lim_t} q_i qs_i for up to two targets (t = 0, 1)
-hipError_t launch_density_acc(const double* q, const double* qs, long long N, double* rho0,
+hipError_t launch_density_acc(const Geom& g, const double* q, const double* qs, double* rho0,
                               int lim0, double w0, double* rho1, int lim1, double w1,
                               hipStream_t st);
 // partial[b] = sum over a chunk of r of W_r * f(r), f = sum_{t<nf} src[t*N + r]
 hipError_t launch_weighted_sum(const Geom& g, const double* src, int nf, double dV,
                                double* partial, int nparts, double* result, hipStream_t st);
 // phi_b = V * rho_b / totden (totden read from device)
-hipError_t launch_phi(const double* rho, int nblk, long long N, double V, const double* totden,
+hipError_t launch_phi(const Geom& g, const double* rho, int nblk, double V, const double* totden,
                       double* phi, hipStream_t st);
 
 hipError_t upload_orientation_tables(const OrientTables& t);

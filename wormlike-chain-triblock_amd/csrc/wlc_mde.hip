@@ -72,6 +72,7 @@ struct wlc_handle {
   wlc_params p{};
   int device = 0;
   long long N = 0, Nk = 0;
+  long long zs = 0, hs = 0;  // padded real-space strides
   int nxh = 0, Ns = 0, Mmax = 0;
   double ds = 0.0;
   int Mb[3] = {0, 0, 0}, Lb[3] = {0, 0, 0}, start[3] = {0, 0, 0}, TX[3] = {1, 1, 1};
@@ -93,6 +94,7 @@ struct wlc_handle {
   bool have_w = false, fwd_done = false, bwd_done = false;
   int generic_zsolve = 0;
   int zs_occ = 1;
+  int hpw = 4;
   // timing
   bool timing = false;
   struct Rec { int k; hipEvent_t a, b; };
@@ -113,7 +115,7 @@ int block_of(const wlc_handle* h, int s) {
 
 double* qs_slot(wlc_handle* h, int s) {
   if (h->p.keep_qstar) return h->qs + h->off[s];
-  return h->qs + (long long)(s & 3) * h->Mmax * h->N;
+  return h->qs + (long long)(s & 3) * h->Mmax * h->hs;
 }
 
 hipEvent_t take_event(wlc_handle* h) {
@@ -161,6 +163,27 @@ struct TimedLaunch {
   }
 };
 
+// dense [nf][nz][ny][nx] <-> padded device layout (plane stride zs, field stride hs)
+hipError_t copy_fields_in(wlc_handle* h, double* dst, const double* src, int nf, hipMemcpyKind kind) {
+  const size_t row = sizeof(double) * (size_t)h->p.nx * h->p.ny;
+  for (int f = 0; f < nf; ++f) {
+    hipError_t e = hipMemcpy2DAsync(dst + (long long)f * h->hs, sizeof(double) * h->zs,
+                                    src + (long long)f * h->N, row, row, h->p.nz, kind, h->st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t copy_fields_out(wlc_handle* h, double* dst, const double* src, int nf, hipMemcpyKind kind) {
+  const size_t row = sizeof(double) * (size_t)h->p.nx * h->p.ny;
+  for (int f = 0; f < nf; ++f) {
+    hipError_t e = hipMemcpy2DAsync(dst + (long long)f * h->N, row, src + (long long)f * h->hs,
+                                    sizeof(double) * h->zs, row, h->p.nz, kind, h->st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 int run_step(wlc_handle* h, int dir, int s) {
   const int Ns = h->Ns;
   const int order = dir > 0 ? std::min(s, 3) : std::min(Ns - s, 3);
@@ -192,8 +215,9 @@ int run_step(wlc_handle* h, int dir, int s) {
     cdelta = 11.0 / 6.0;
   }
   ra.ds = h->ds;
-  ra.w = h->w + (long long)blk * h->N;
+  ra.w = h->w + (long long)blk * h->hs;
   ra.M = M;
+  ra.hpw = h->hpw;
   ra.spec = h->spec;
   {
     TimedLaunch t(h, 0);
@@ -238,14 +262,14 @@ int density_at(wlc_handle* h, int s) {
   for (int b = 0; b < h->p.nblk && n < 2; ++b) {
     const int j = s - h->start[b];
     if (j < 0 || j > h->p.Nblk[b]) continue;
-    rho[n] = h->rho + (long long)b * h->N;
+    rho[n] = h->rho + (long long)b * h->hs;
     lim[n] = std::min(Ms, h->Mb[b]);
     wt[n] = h->cw[b][j];
     ++n;
   }
   if (n == 0) return WLC_OK;
   TimedLaunch t(h, 3);
-  HIPCHK(wlc::launch_density_acc(h->q + h->off[s], qs_slot(h, s), h->N, rho[0], lim[0], wt[0],
+  HIPCHK(wlc::launch_density_acc(h->geom, h->q + h->off[s], qs_slot(h, s), rho[0], lim[0], wt[0],
                                  rho[1], lim[1], wt[1], h->st));
   return WLC_OK;
 }
@@ -325,6 +349,7 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   h->p = *p;
   if (const char* ge = std::getenv("WLC_GENERIC_ZSOLVE")) h->generic_zsolve = std::atoi(ge) != 0;
   if (const char* oc = std::getenv("WLC_ZSOLVE_OCC")) h->zs_occ = std::atoi(oc) == 2 ? 2 : 1;
+  if (const char* hp = std::getenv("WLC_HPW")) h->hpw = std::max(1, std::atoi(hp));
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
     return fail(WLC_EHIP, "no HIP device");
@@ -341,6 +366,9 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     }
   }
   h->N = (long long)p->nx * p->ny * p->nz;
+  // padded real-space strides (see layout notes in wlc_kernels.h)
+  h->zs = (long long)p->nx * p->ny + 16;
+  h->hs = (long long)p->nz * h->zs + 32;
   h->nxh = p->nx / 2 + 1;
   h->Nk = (long long)h->nxh * p->ny * p->nz;
   h->Ns = p->Ns;
@@ -359,7 +387,7 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   }
   h->off.resize(h->Ns + 2);
   h->off[0] = 0;
-  for (int s = 0; s <= h->Ns; ++s) h->off[s + 1] = h->off[s] + (long long)h->Mb[block_of(h, s)] * h->N;
+  for (int s = 0; s <= h->Ns; ++s) h->off[s + 1] = h->off[s] + (long long)h->Mb[block_of(h, s)] * h->hs;
   const int edges[3] = {p->nx, p->ny, p->nz};
   for (int a = 0; a < 3; ++a) {
     std::string err = wlc::make_axis_plan(edges[a], &h->plan[a]);
@@ -379,11 +407,11 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   }
   ALLOC(h->q, sizeof(double) * (size_t)h->off[h->Ns + 1]);
   if (p->keep_qstar) ALLOC(h->qs, sizeof(double) * (size_t)h->off[h->Ns + 1]);
-  else ALLOC(h->qs, sizeof(double) * (size_t)4 * h->Mmax * h->N);
-  ALLOC(h->spec, sizeof(double2) * (size_t)h->Mmax * h->Nk);
-  ALLOC(h->w, sizeof(double) * (size_t)p->nblk * h->N);
-  ALLOC(h->rho, sizeof(double) * (size_t)p->nblk * h->N);
-  ALLOC(h->phi, sizeof(double) * (size_t)p->nblk * h->N);
+  else ALLOC(h->qs, sizeof(double) * (size_t)4 * h->Mmax * h->hs);
+  ALLOC(h->spec, sizeof(double2) * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz);
+  ALLOC(h->w, sizeof(double) * (size_t)p->nblk * h->hs);
+  ALLOC(h->rho, sizeof(double) * (size_t)p->nblk * h->hs);
+  ALLOC(h->phi, sizeof(double) * (size_t)p->nblk * h->hs);
   ALLOC(h->partial, sizeof(double) * h->nparts);
   ALLOC(h->scal, sizeof(double) * 4);
   for (int a = 0; a < 3; ++a) {
@@ -396,6 +424,9 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
 #undef ALLOC
   wlc::Geom& g = h->geom;
   g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nxh = h->nxh;
+  g.nt = wlc::kx_tiles(h->nxh);
+  g.zs = h->zs;
+  g.hs = h->hs;
   wlc::AxisDev* ax[3] = {&g.ax, &g.ay, &g.az};
   for (int a = 0; a < 3; ++a) {
     ax[a]->n = h->plan[a].n;
@@ -422,7 +453,7 @@ int wlc_mde_set_box(wlc_handle* h, const double box[3]) {
 int wlc_mde_set_w(wlc_handle* h, const double* w_host) {
   if (!h || !w_host) return fail(WLC_EINVAL, "NULL argument");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemcpyAsync(h->w, w_host, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyHostToDevice, h->st));
+  HIPCHK(copy_fields_in(h, h->w, w_host, h->p.nblk, hipMemcpyHostToDevice));
   HIPCHK(hipStreamSynchronize(h->st));
   h->have_w = true;
   h->fwd_done = h->bwd_done = false;
@@ -432,7 +463,7 @@ int wlc_mde_set_w(wlc_handle* h, const double* w_host) {
 int wlc_mde_set_w_device(wlc_handle* h, const double* w_dev) {
   if (!h || !w_dev) return fail(WLC_EINVAL, "NULL argument");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemcpyAsync(h->w, w_dev, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(copy_fields_in(h, h->w, w_dev, h->p.nblk, hipMemcpyDeviceToDevice));
   h->have_w = true;
   h->fwd_done = h->bwd_done = false;
   return WLC_OK;
@@ -442,7 +473,7 @@ int wlc_mde_forward(wlc_handle* h) {
   if (!h) return fail(WLC_EINVAL, "NULL handle");
   if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede forward");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(wlc::launch_set_delta(h->q, h->Mb[block_of(h, 0)], h->N, h->st));
+  HIPCHK(wlc::launch_set_delta(h->geom, h->q, h->Mb[block_of(h, 0)], h->st));
   for (int s = 1; s <= h->Ns; ++s) {
     int rc = run_step(h, +1, s);
     if (rc) return rc;
@@ -458,8 +489,8 @@ int wlc_mde_backward(wlc_handle* h) {
   if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede backward");
   HIPCHK(hipSetDevice(h->device));
   const int Ns = h->Ns;
-  HIPCHK(wlc::launch_fill(h->rho, 0.0, (long long)h->p.nblk * h->N, h->st));
-  HIPCHK(wlc::launch_set_delta(qs_slot(h, Ns), h->Mb[block_of(h, Ns)], h->N, h->st));
+  HIPCHK(wlc::launch_fill(h->rho, 0.0, (long long)h->p.nblk * h->hs, h->st));
+  HIPCHK(wlc::launch_set_delta(h->geom, qs_slot(h, Ns), h->Mb[block_of(h, Ns)], h->st));
   const bool dens = h->fwd_done;
   if (dens) {
     int rc = density_at(h, Ns);
@@ -485,7 +516,7 @@ static int density_common(wlc_handle* h, double* Q, double* totden) {
   const double V = h->p.box[0] * h->p.box[1] * h->p.box[2];
   HIPCHK(wlc::launch_weighted_sum(h->geom, h->q + h->off[h->Ns], 1, dV, h->partial, h->nparts, h->scal, h->st));
   HIPCHK(wlc::launch_weighted_sum(h->geom, h->rho, h->p.nblk, dV, h->partial, h->nparts, h->scal + 1, h->st));
-  HIPCHK(wlc::launch_phi(h->rho, h->p.nblk, h->N, V, h->scal + 1, h->phi, h->st));
+  HIPCHK(wlc::launch_phi(h->geom, h->rho, h->p.nblk, V, h->scal + 1, h->phi, h->st));
   double sc[2];
   HIPCHK(hipMemcpyAsync(sc, h->scal, sizeof(sc), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -501,7 +532,8 @@ int wlc_mde_density(wlc_handle* h, double* phi_host, double* Q, double* totden) 
   int rc = density_common(h, Q, totden);
   if (rc) return rc;
   if (phi_host) {
-    HIPCHK(hipMemcpy(phi_host, h->phi, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyDeviceToHost));
+    HIPCHK(copy_fields_out(h, phi_host, h->phi, h->p.nblk, hipMemcpyDeviceToHost));
+    HIPCHK(hipStreamSynchronize(h->st));
     for (long long e = 0; e < (long long)h->p.nblk * h->N; ++e)
       if (!std::isfinite(phi_host[e])) return fail(WLC_ENONFINITE, "phi not finite");
   }
@@ -512,7 +544,7 @@ int wlc_mde_density_device(wlc_handle* h, double* phi_dev, double* Q, double* to
   if (!h) return fail(WLC_EINVAL, "NULL handle");
   int rc = density_common(h, Q, totden);
   if (rc) return rc;
-  if (phi_dev) HIPCHK(hipMemcpyAsync(phi_dev, h->phi, sizeof(double) * (size_t)h->p.nblk * h->N, hipMemcpyDeviceToDevice, h->st));
+  if (phi_dev) HIPCHK(copy_fields_out(h, phi_dev, h->phi, h->p.nblk, hipMemcpyDeviceToDevice));
   return WLC_OK;
 }
 
@@ -530,7 +562,8 @@ int wlc_mde_get_q(wlc_handle* h, int dir, int s, double* q_host) {
     src = qs_slot(h, s);
   }
   HIPCHK(hipStreamSynchronize(h->st));
-  HIPCHK(hipMemcpy(q_host, src, sizeof(double) * (size_t)(M * h->N), hipMemcpyDeviceToHost));
+  HIPCHK(copy_fields_out(h, q_host, src, (int)M, hipMemcpyDeviceToHost));
+  HIPCHK(hipStreamSynchronize(h->st));
   return WLC_OK;
 }
 

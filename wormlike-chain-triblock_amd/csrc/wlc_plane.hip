@@ -19,129 +19,171 @@ namespace wlc {
 namespace {
 constexpr int kThreads = 256;
 constexpr int kChunk = 8;  // plane elements each thread loads per batch
+constexpr int kPairs = 4;  // x pairs each thread loads per batch (kernel A)
 }  // namespace
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
 // Remap the linear block id so that every XCD walks a contiguous range of
-// (harmonic, z-plane) pairs, z fastest: the z-fastest half spectrum then has the
-// 16-byte pieces of each cache line written/read by workgroups of one XCD at
+// (harmonic group, z-plane) pairs, z fastest: the z-fastest half spectrum then has
+// the 16-byte pieces of each cache line written/read by workgroups of one XCD at
 // about the same time.
-__device__ __forceinline__ void plane_of_block(int nz, int& z, int& i) {
+__device__ __forceinline__ void plane_of_block(int nz, int& z, int& grp) {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
   const int q = nb >> 3;
   const int logical = (b < (q << 3)) ? ((b & 7) * q + (b >> 3)) : b;
-  i = logical / nz;
-  z = logical - i * nz;
+  grp = logical / nz;
+  z = logical - grp * nz;
 }
 
-template <int MAXE, int ORDER>
-__global__ void __launch_bounds__(kThreads) k_rhs_fft_xy(Geom g, RhsArgs a) {
+// x then y forward transform of the packed plane in LDS (sizes compile-time when
+// NX, NY are nonzero)
+template <int NX, int NY, int MAXE>
+__device__ __forceinline__ void plane_fft_x(double2* lds, const Geom& g) {
+  if constexpr (NX > 0) FftCt<-1, NX, NY / 2, 1, NX, kThreads, true>::run(lds, g.ax.tw);
+  else fft_axis<-1, MAXE>(lds, g.ax, g.ny / 2, 1, g.nx);
+}
+template <int NX, int NY, int MAXE, int DIR>
+__device__ __forceinline__ void plane_fft_y(double2* lds, const Geom& g) {
+  if constexpr (NX > 0) FftCt<DIR, NY, NX / 2 + 1, NX / 2 + 1, 1, kThreads, true>::run(lds, g.ay.tw);
+  else fft_axis<DIR, MAXE>(lds, g.ay, g.nxh, g.nxh, 1);
+}
+template <int NX, int NY, int MAXE>
+__device__ __forceinline__ void plane_ifft_x(double2* lds, const Geom& g) {
+  if constexpr (NX > 0) FftCt<+1, NX, NY / 2, 1, NX, kThreads, true>::run(lds, g.ax.tw);
+  else fft_axis<+1, MAXE>(lds, g.ax, g.ny / 2, 1, g.nx);
+}
+
+// LDS index of logical complex slot u (padded for the compile-time sizes)
+template <int NX>
+__device__ __forceinline__ int lp(int u) { return NX > 0 ? lds_pad(u) : u; }
+
+// A: per (z-plane, group of hpw harmonics).  The w plane is read once per workgroup.
+template <int NX, int NY, int MAXE, int ORDER>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1))) k_rhs_fft_xy(Geom g, RhsArgs a) {
   extern __shared__ double2 lds[];
-  int z, i;
-  plane_of_block(g.nz, z, i);
-  const int nx = g.nx, ny = g.ny, nxh = g.nxh;
-  const long long N = (long long)nx * ny * g.nz;
-  const long long plane = (long long)z * nx * ny;
-  const bool u1 = i < a.hM[0];
-  const bool u2 = ORDER >= 2 && i < a.hM[1];
-  const bool u3 = ORDER >= 3 && i < a.hM[2];
-  const double* q1 = u1 ? a.h[0] + i * N + plane : nullptr;
-  const double* q2 = u2 ? a.h[1] + i * N + plane : nullptr;
-  const double* q3 = u3 ? a.h[2] + i * N + plane : nullptr;
+  int z, grp;
+  plane_of_block(g.nz, z, grp);
+  const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
+  const int nxh = nx / 2 + 1;
+  const int nz = g.nz;
+  const long long hs = g.hs;
+  const long long plane = (long long)z * g.zs;
+  const int tot = nx * ny;
   const double* w = a.w + plane;
   double* pk = reinterpret_cast<double*>(lds);  // packed rows: (row 2j, row 2j+1) -> complex
   const double ds = a.ds;
   const double a0 = a.a[0], a1 = a.a[1], a2 = a.a[2];
   const double b0 = a.b[0], b1 = a.b[1], b2 = a.b[2];
-  const int tot = nx * ny;
-  for (int base = 0; base < tot; base += kChunk * kThreads) {
-    double v1[kChunk], v2[kChunk], v3[kChunk], ww[kChunk];
+  const int i0 = grp * a.hpw;
+  const int i1 = min(i0 + a.hpw, a.M);
+  const double2* w2 = reinterpret_cast<const double2*>(w);
+  const int tot2 = tot >> 1;  // nx is even: x pairs (2p, 2p+1) share a row
+  for (int i = i0; i < i1; ++i) {
+    const bool u1 = i < a.hM[0];
+    const bool u2 = ORDER >= 2 && i < a.hM[1];
+    const bool u3 = ORDER >= 3 && i < a.hM[2];
+    const double2* q1 = reinterpret_cast<const double2*>(u1 ? a.h[0] + i * hs + plane : w);
+    const double2* q2 = reinterpret_cast<const double2*>(u2 ? a.h[1] + i * hs + plane : w);
+    const double2* q3 = reinterpret_cast<const double2*>(u3 ? a.h[2] + i * hs + plane : w);
+    for (int base = 0; base < tot2; base += kPairs * kThreads) {
+      double2 v1[kPairs], v2[kPairs], v3[kPairs], ww[kPairs];
+      const double2 zero = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int c = 0; c < kChunk; ++c) {
-      const int e = base + c * kThreads + threadIdx.x;
-      const bool in = e < tot;
-      v1[c] = (u1 && in) ? q1[e] : 0.0;
-      v2[c] = (u2 && in) ? q2[e] : 0.0;
-      v3[c] = (u3 && in) ? q3[e] : 0.0;
-      ww[c] = in ? w[e] : 0.0;
-    }
-#pragma unroll
-    for (int c = 0; c < kChunk; ++c) {
-      const int e = base + c * kThreads + threadIdx.x;
-      if (e < tot) {
-        const double t = a0 * v1[c] + a1 * v2[c] + a2 * v3[c];
-        const double u = b0 * v1[c] + b1 * v2[c] + b2 * v3[c];
-        const int y = e / nx, x = e - y * nx;
-        pk[2 * ((y >> 1) * nx + x) + (y & 1)] = t - (ds * ww[c]) * u;
+      for (int c = 0; c < kPairs; ++c) {
+        const int e = base + c * kThreads + threadIdx.x;
+        const bool in = e < tot2;
+        v1[c] = (u1 && in) ? q1[e] : zero;
+        v2[c] = (u2 && in) ? q2[e] : zero;
+        v3[c] = (u3 && in) ? q3[e] : zero;
+        ww[c] = in ? w2[e] : zero;
       }
-    }
-  }
-  __syncthreads();
-  fft_axis<-1, MAXE>(lds, g.ax, ny / 2, 1, nx);
-  // unpack Z = A + iB into A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i)
-  {
-    constexpr int UPT = (MAXE + 1) / 2 + 1;
-    double2 ra[UPT], rb[UPT];
-    int dst[UPT];
-    const int tu = (ny / 2) * nxh;
 #pragma unroll
-    for (int c = 0; c < UPT; ++c) {
-      const int idx = threadIdx.x + c * kThreads;
-      dst[c] = -1;
-      if (idx < tu) {
-        const int j = idx / nxh, k = idx - j * nxh;
-        const double2 zk = lds[j * nx + k];
-        const double2 zn = lds[j * nx + (k == 0 ? 0 : nx - k)];
-        ra[c] = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
-        rb[c] = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
-        dst[c] = (2 * j) * nxh + k;
+      for (int c = 0; c < kPairs; ++c) {
+        const int e = base + c * kThreads + threadIdx.x;
+        if (e < tot2) {
+          const int y = (2 * e) / nx, x = 2 * e - y * nx;
+          const double t0 = a0 * v1[c].x + a1 * v2[c].x + a2 * v3[c].x;
+          const double s0 = b0 * v1[c].x + b1 * v2[c].x + b2 * v3[c].x;
+          const double t1 = a0 * v1[c].y + a1 * v2[c].y + a2 * v3[c].y;
+          const double s1 = b0 * v1[c].y + b1 * v2[c].y + b2 * v3[c].y;
+          const int u = (y >> 1) * nx + x;
+          pk[2 * lp<NX>(u) + (y & 1)] = t0 - (ds * ww[c].x) * s0;
+          pk[2 * lp<NX>(u + 1) + (y & 1)] = t1 - (ds * ww[c].y) * s1;
+        }
       }
     }
     __syncthreads();
+    plane_fft_x<NX, NY, MAXE>(lds, g);
+    // unpack Z = A + iB into A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i)
+    {
+      constexpr int UPT = NX > 0 ? ((NY / 2) * (NX / 2 + 1) + kThreads - 1) / kThreads
+                                 : (MAXE + 1) / 2 + 1;
+      double2 ra[UPT], rb[UPT];
+      int dst[UPT];
+      const int tu = (ny / 2) * nxh;
 #pragma unroll
-    for (int c = 0; c < UPT; ++c)
-      if (dst[c] >= 0) {
-        lds[dst[c]] = ra[c];
-        lds[dst[c] + nxh] = rb[c];
+      for (int c = 0; c < UPT; ++c) {
+        const int idx = threadIdx.x + c * kThreads;
+        dst[c] = -1;
+        if (idx < tu) {
+          const int j = idx / nxh, k = idx - j * nxh;
+          const double2 zk = lds[lp<NX>(j * nx + k)];
+          const double2 zn = lds[lp<NX>(j * nx + (k == 0 ? 0 : nx - k))];
+          ra[c] = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+          rb[c] = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
+          dst[c] = (2 * j) * nxh + k;
+        }
       }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < UPT; ++c)
+        if (dst[c] >= 0) {
+          lds[lp<NX>(dst[c])] = ra[c];
+          lds[lp<NX>(dst[c] + nxh)] = rb[c];
+        }
+      __syncthreads();
+    }
+    plane_fft_y<NX, NY, MAXE, -1>(lds, g);
+    // half spectrum, 64-byte runs of 4 kx per (ky, tile, z)
+    for (int e = threadIdx.x; e < ny * nxh; e += kThreads) {
+      const int ky = e / nxh, kx = e - ky * nxh;
+      a.spec[spec_index(i, ky, kx, z, ny, g.nt, nz)] = lds[lp<NX>(e)];
+    }
     __syncthreads();
   }
-  fft_axis<-1, MAXE>(lds, g.ay, nxh, nxh, 1);
-  // half spectrum [i][ky][kx][z]
-  const int nz = g.nz;
-  double2* out = a.spec + (long long)i * ny * nxh * nz + z;
-  for (int e = threadIdx.x; e < ny * nxh; e += kThreads) out[(long long)e * nz] = lds[e];
 }
 
-template <int MAXE>
+// C: per (z-plane, harmonic)
+template <int NX, int NY, int MAXE>
 __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
   extern __shared__ double2 lds[];
   int z, i;
   plane_of_block(g.nz, z, i);
-  const int nx = g.nx, ny = g.ny, nxh = g.nxh;
+  const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
+  const int nxh = nx / 2 + 1;
   const int nz = g.nz;
-  const double2* in = a.spec + (long long)i * ny * nxh * nz + z;
   const int tin = ny * nxh;
   for (int base = 0; base < tin; base += kChunk * kThreads) {
     double2 r[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; ++c) {
       const int e = base + c * kThreads + threadIdx.x;
-      r[c] = e < tin ? in[(long long)e * nz] : make_double2(0.0, 0.0);
+      const int ky = e / nxh, kx = e - ky * nxh;
+      r[c] = e < tin ? a.spec[spec_index(i, ky, kx, z, ny, g.nt, nz)] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int c = 0; c < kChunk; ++c) {
       const int e = base + c * kThreads + threadIdx.x;
-      if (e < tin) lds[e] = r[c];
+      if (e < tin) lds[lp<NX>(e)] = r[c];
     }
   }
   __syncthreads();
-  fft_axis<+1, MAXE>(lds, g.ay, nxh, nxh, 1);
+  plane_fft_y<NX, NY, MAXE, +1>(lds, g);
   // pack rows 2j, 2j+1 (Hermitian-extended along x; imaginary parts of kx = 0 and
   // kx = nx/2 dropped) into Z = A + iB
   {
-    constexpr int UPT = MAXE + 1;
+    constexpr int UPT = NX > 0 ? ((NY / 2) * NX + kThreads - 1) / kThreads : MAXE + 1;
     double2 rz[UPT];
     int dst[UPT];
     const int tot = (ny / 2) * nx;
@@ -154,15 +196,15 @@ __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
         const int j = idx / nx, k = idx - j * nx;
         double2 A, B;
         if (k <= half) {
-          A = lds[(2 * j) * nxh + k];
-          B = lds[(2 * j + 1) * nxh + k];
+          A = lds[lp<NX>((2 * j) * nxh + k)];
+          B = lds[lp<NX>((2 * j + 1) * nxh + k)];
           if (k == 0 || k == half) {
             A.y = 0.0;
             B.y = 0.0;
           }
         } else {
-          A = lds[(2 * j) * nxh + nx - k];
-          B = lds[(2 * j + 1) * nxh + nx - k];
+          A = lds[lp<NX>((2 * j) * nxh + nx - k)];
+          B = lds[lp<NX>((2 * j + 1) * nxh + nx - k)];
           A.y = -A.y;
           B.y = -B.y;
         }
@@ -173,17 +215,16 @@ __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < UPT; ++c)
-      if (dst[c] >= 0) lds[dst[c]] = rz[c];
+      if (dst[c] >= 0) lds[lp<NX>(dst[c])] = rz[c];
     __syncthreads();
   }
-  fft_axis<+1, MAXE>(lds, g.ax, ny / 2, 1, nx);
-  const long long N = (long long)nx * ny * g.nz;
-  double* out = a.out + i * N + (long long)z * nx * ny;
+  plane_ifft_x<NX, NY, MAXE>(lds, g);
+  double* out = a.out + i * g.hs + (long long)z * g.zs;
   const double* pk = reinterpret_cast<const double*>(lds);
   const double sc = a.scale;
   for (int e = threadIdx.x; e < nx * ny; e += kThreads) {
     const int y = e / nx, x = e - y * nx;
-    out[e] = pk[2 * ((y >> 1) * nx + x) + (y & 1)] * sc;
+    out[e] = pk[2 * lp<NX>((y >> 1) * nx + x) + (y & 1)] * sc;
   }
 }
 
@@ -198,46 +239,63 @@ int pick_maxe(long long elems, int nthr) {
 }
 }  // namespace
 
-template <int MAXE>
+template <int NX, int NY, int MAXE>
 static void* rhs_kernel(int order) {
   switch (order) {
-    case 1: return (void*)k_rhs_fft_xy<MAXE, 1>;
-    case 2: return (void*)k_rhs_fft_xy<MAXE, 2>;
-    case 3: return (void*)k_rhs_fft_xy<MAXE, 3>;
+    case 1: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 1>;
+    case 2: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 2>;
+    case 3: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 3>;
   }
   return nullptr;
 }
 
+// square planes with compile-time sizes; everything else takes the runtime kernels
+static bool specialised(const Geom& g) {
+  return g.nx == g.ny && (g.nx == 32 || g.nx == 64);
+}
+
+static size_t plane_lds_bytes(const Geom& g) {
+  const size_t n = (size_t)g.ny * g.nxh;
+  return sizeof(double2) * (specialised(g) ? n + n / 8 + 1 : n);
+}
+
 hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_t st) {
-  const size_t lds = sizeof(double2) * (size_t)g.ny * g.nxh;
-  const int me = pick_maxe((long long)g.ny * g.nxh, kThreads);
+  const size_t lds = plane_lds_bytes(g);
   void* k = nullptr;
-  switch (me) {
-    case 8: k = rhs_kernel<8>(order); break;
-    case 16: k = rhs_kernel<16>(order); break;
-    case 32: k = rhs_kernel<32>(order); break;
-    case 64: k = rhs_kernel<64>(order); break;
-    default: return hipErrorInvalidValue;
+  if (specialised(g)) {
+    k = g.nx == 32 ? rhs_kernel<32, 32, 0>(order) : rhs_kernel<64, 64, 0>(order);
+  } else {
+    switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
+      case 8: k = rhs_kernel<0, 0, 8>(order); break;
+      case 16: k = rhs_kernel<0, 0, 16>(order); break;
+      case 32: k = rhs_kernel<0, 0, 32>(order); break;
+      case 64: k = rhs_kernel<0, 0, 64>(order); break;
+      default: return hipErrorInvalidValue;
+    }
   }
-  if (!k) return hipErrorInvalidValue;
+  if (!k || a.hpw < 1) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   Geom gg = g;
   RhsArgs aa = a;
   void* args[] = {&gg, &aa};
-  return hipLaunchKernel(k, dim3(g.nz * a.M), dim3(kThreads), args, lds, st);
+  const int groups = (a.M + a.hpw - 1) / a.hpw;
+  return hipLaunchKernel(k, dim3(g.nz * groups), dim3(kThreads), args, lds, st);
 }
 
 hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st) {
-  const size_t lds = sizeof(double2) * (size_t)g.ny * g.nxh;
-  const int me = pick_maxe((long long)g.ny * g.nxh, kThreads);
+  const size_t lds = plane_lds_bytes(g);
   void* k = nullptr;
-  switch (me) {
-    case 8: k = (void*)k_ifft_xy<8>; break;
-    case 16: k = (void*)k_ifft_xy<16>; break;
-    case 32: k = (void*)k_ifft_xy<32>; break;
-    case 64: k = (void*)k_ifft_xy<64>; break;
-    default: return hipErrorInvalidValue;
+  if (specialised(g)) {
+    k = g.nx == 32 ? (void*)k_ifft_xy<32, 32, 0> : (void*)k_ifft_xy<64, 64, 0>;
+  } else {
+    switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
+      case 8: k = (void*)k_ifft_xy<0, 0, 8>; break;
+      case 16: k = (void*)k_ifft_xy<0, 0, 16>; break;
+      case 32: k = (void*)k_ifft_xy<0, 0, 32>; break;
+      case 64: k = (void*)k_ifft_xy<0, 0, 64>; break;
+      default: return hipErrorInvalidValue;
+    }
   }
   hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -250,17 +308,20 @@ hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // elementwise / reduction kernels
 // ---------------------------------------------------------------------------
-__global__ void k_set_delta(double* s, int M, long long N) {
-  const long long tot = (long long)M * N;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
-       e += (long long)gridDim.x * blockDim.x)
-    s[e] = (e < N) ? 1.0 : 0.0;
+// real-space kernels visit the valid points of the padded layout: grid (blocks over
+// the nx*ny plane, nz * planes-per-field)
+__global__ void k_set_delta(Geom g, double* s, int M) {
+  const int nxy = g.nx * g.ny;
+  const int z = blockIdx.y % g.nz, i = blockIdx.y / g.nz;
+  double* p = s + i * g.hs + (long long)z * g.zs;
+  const double v = i == 0 ? 1.0 : 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nxy; e += gridDim.x * blockDim.x) p[e] = v;
 }
 
-hipError_t launch_set_delta(double* slice, int M, long long N, hipStream_t st) {
-  const long long tot = (long long)M * N;
-  int blocks = (int)std::min<long long>((tot + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_set_delta, dim3(blocks), dim3(256), 0, st, slice, M, N);
+hipError_t launch_set_delta(const Geom& g, double* slice, int M, hipStream_t st) {
+  const int nxy = g.nx * g.ny;
+  dim3 grid((nxy + 255) / 256, g.nz * M);
+  hipLaunchKernelGGL(k_set_delta, grid, dim3(256), 0, st, g, slice, M);
   return hipGetLastError();
 }
 
@@ -276,30 +337,33 @@ hipError_t launch_fill(double* dst, double v, long long n, hipStream_t st) {
   return hipGetLastError();
 }
 
-__global__ void k_density_acc(const double* __restrict__ q, const double* __restrict__ qs,
-                              long long N, double* rho0, int lim0, double w0, double* rho1,
-                              int lim1, double w1) {
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N;
-       r += (long long)gridDim.x * blockDim.x) {
-    double s = 0.0, s0 = 0.0, s1 = 0.0;
-    const int top = lim0 > lim1 ? lim0 : lim1;
-    for (int i = 0; i < top; ++i) {
-      s += q[i * N + r] * qs[i * N + r];
-      if (i + 1 == lim0) s0 = s;
-      if (i + 1 == lim1) s1 = s;
-    }
-    if (rho0) rho0[r] += w0 * s0;
-    if (rho1) rho1[r] += w1 * s1;
+// rho_t += w_t * sum_{i < lim_t} q_i qs_i at every grid point, for up to two blocks
+__global__ void k_density_acc(Geom g, const double* __restrict__ q, const double* __restrict__ qs,
+                              double* rho0, int lim0, double w0, double* rho1, int lim1, double w1) {
+  const int nxy = g.nx * g.ny;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nxy) return;
+  const long long r = (long long)blockIdx.y * g.zs + e;
+  const long long hs = g.hs;
+  double s = 0.0, s0 = 0.0, s1 = 0.0;
+  const int top = lim0 > lim1 ? lim0 : lim1;
+  for (int i = 0; i < top; ++i) {
+    s += q[i * hs + r] * qs[i * hs + r];
+    if (i + 1 == lim0) s0 = s;
+    if (i + 1 == lim1) s1 = s;
   }
+  if (rho0) rho0[r] += w0 * s0;
+  if (rho1) rho1[r] += w1 * s1;
 }
 
-hipError_t launch_density_acc(const double* q, const double* qs, long long N, double* rho0,
+hipError_t launch_density_acc(const Geom& g, const double* q, const double* qs, double* rho0,
                               int lim0, double w0, double* rho1, int lim1, double w1,
                               hipStream_t st) {
   if (!rho1) lim1 = 0;
-  int blocks = (int)std::min<long long>((N + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_density_acc, dim3(blocks), dim3(256), 0, st, q, qs, N, rho0, lim0, w0,
-                     rho1, lim1, w1);
+  const int nxy = g.nx * g.ny;
+  dim3 grid((nxy + 255) / 256, g.nz);
+  hipLaunchKernelGGL(k_density_acc, grid, dim3(256), 0, st, g, q, qs, rho0, lim0, w0, rho1,
+                     lim1, w1);
   return hipGetLastError();
 }
 
@@ -316,8 +380,9 @@ __global__ void k_weighted_partial(Geom g, const double* __restrict__ src, int n
     const int x = (int)(r % g.nx);
     const long long t = r / g.nx;
     const int y = (int)(t % g.ny), z = (int)(t / g.ny);
+    const long long pr = (long long)z * g.zs + (long long)y * g.nx + x;
     double f = 0.0;
-    for (int k = 0; k < nf; ++k) f += src[k * N + r];
+    for (int k = 0; k < nf; ++k) f += src[k * g.hs + pr];
     acc += simp_w(x) * simp_w(y) * simp_w(z) * f;
   }
   red[threadIdx.x] = acc;
@@ -351,19 +416,20 @@ hipError_t launch_weighted_sum(const Geom& g, const double* src, int nf, double 
   return hipGetLastError();
 }
 
-__global__ void k_phi(const double* __restrict__ rho, long long tot, double V,
+__global__ void k_phi(Geom g, const double* __restrict__ rho, double V,
                       const double* __restrict__ totden, double* phi) {
-  const double f = V / *totden;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
-       e += (long long)gridDim.x * blockDim.x)
-    phi[e] = rho[e] * f;
+  const int nxy = g.nx * g.ny;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nxy) return;
+  const long long r = (long long)(blockIdx.y % g.nz) * g.zs + (blockIdx.y / g.nz) * g.hs + e;
+  phi[r] = rho[r] * (V / *totden);
 }
 
-hipError_t launch_phi(const double* rho, int nblk, long long N, double V, const double* totden,
+hipError_t launch_phi(const Geom& g, const double* rho, int nblk, double V, const double* totden,
                       double* phi, hipStream_t st) {
-  const long long tot = (long long)nblk * N;
-  int blocks = (int)std::min<long long>((tot + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_phi, dim3(blocks), dim3(256), 0, st, rho, tot, V, totden, phi);
+  const int nxy = g.nx * g.ny;
+  dim3 grid((nxy + 255) / 256, g.nz * nblk);
+  hipLaunchKernelGGL(k_phi, grid, dim3(256), 0, st, g, rho, V, totden, phi);
   return hipGetLastError();
 }
 

@@ -5,8 +5,8 @@
 // inverse z pass.
 //
 // One thread owns one k-point (kx, ky, kz) and holds all M harmonics of it in
-// registers.  The half spectrum is laid out [i][ky][kx][z] (z fastest), so a
-// column of one harmonic is contiguous.
+// registers.  The half spectrum is laid out [i][ky][kx/4][z][kx%4]: a tile of four
+// kx columns of one harmonic is contiguous (4 nz complex values).
 #include <hip/hip_runtime.h>
 
 #include "wlc_device.h"
@@ -56,9 +56,8 @@ __global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
   const int tx = t / nz, z = t - tx * nz;
   const int kx = blockIdx.x * TX + tx, ky = blockIdx.y;
   const bool valid = kx < nxh;
-  // spectrum layout [i][ky][kx][z]
-  const long long istride = (long long)ny * nxh * nz;
-  double2* p = a.spec + ((long long)ky * nxh + (valid ? kx : 0)) * nz + z;
+  const long long istride = spec_index(1, 0, 0, 0, ny, g.nt, nz);
+  double2* p = a.spec + spec_index(0, ky, valid ? kx : 0, z, ny, g.nt, nz);
   double2 v[kM];
 #pragma unroll
   for (int i = 0; i < kM; ++i) v[i] = valid ? p[i * istride] : make_double2(0.0, 0.0);
@@ -79,10 +78,7 @@ __global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
 
 // kernel B for nz = 64: one wave owns one (kx, ky) column, lane = z; the 64-point z
 // transform is an 8 x 8 four-step done wave-locally in LDS, eight harmonics per
-// batch (no workgroup barriers); each lane's step-1 twiddles stay in registers.
-// Forward: step 1 reads the column straight from global memory (next batch
-// prefetched), the result lands in v[] (lane = kz).  Inverse: v[] -> LDS -> step 1
-// -> step 2, whose outputs go straight to global memory.  LDS rows of 8 are padded
+// batch; each lane's step-1 twiddles stay in registers.  LDS rows of 8 are padded
 // to 9 so the strided step-2 reads are bank-conflict free.
 __device__ __forceinline__ int pad8(int u) { return u + (u >> 3); }
 constexpr int kPadBatch = 72;  // padded units per harmonic (64 + 8)
@@ -93,32 +89,50 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void z64_load_batch(double2 (&u)[8], const double2* __restrict__ col,
-                                               long long istride, int b0, int h, int j, bool ok) {
-  const bool in = ok && (b0 + h < kM);
-  const double2* src = col + (long long)(in ? b0 + h : 0) * istride + j;
+// Workgroup = one kx tile (4 columns) of one ky row; wave w owns column kx = 4 t + w.
+// Per batch of 8 harmonics the four waves cooperatively load each harmonic's 4 KB tile
+// (64 z x 4 kx, contiguous) with 1 KB-coalesced loads into LDS, wave w's region holding
+// its column in natural z order; the next batch is prefetched into registers meanwhile.
+// The inverse writes the step-2 outputs back into the regions and the workgroup stores
+// the tiles with coalesced 1 KB stores.
+constexpr int kRegion = kHB * kPadBatch + 2;  // per-wave LDS region (padded stride)
+
+__device__ __forceinline__ void tile_load(double2 (&r)[kHB], const double2* __restrict__ tile,
+                                          long long istride, int b0, int t) {
 #pragma unroll
-  for (int n2 = 0; n2 < 8; ++n2) u[n2] = in ? src[8 * n2] : make_double2(0.0, 0.0);
+  for (int h = 0; h < kHB; ++h)
+    r[h] = (b0 + h < kM) ? tile[(long long)(b0 + h) * istride + t] : make_double2(0.0, 0.0);
 }
 
-__device__ __forceinline__ void z64_forward(double2 (&v)[kM], double2* wl, int lane,
-                                            const double2 (&tw)[8], const double2* __restrict__ col,
-                                            long long istride, bool ok) {
-  const int h = lane >> 3, j = lane & 7;
-  double2 u[8], nxt[8];
-  z64_load_batch(nxt, col, istride, 0, h, j, ok);
+__device__ __forceinline__ void tile_to_lds(const double2 (&r)[kHB], double2* lds, int t) {
+  const int z = t >> 2, w = t & 3;
+#pragma unroll
+  for (int h = 0; h < kHB; ++h) lds[w * kRegion + h * kPadBatch + pad8(z)] = r[h];
+}
+
+__device__ __forceinline__ void z64_forward(double2 (&v)[kM], double2* lds, int wv, int lane,
+                                            const double2 (&tw)[8], const double2* __restrict__ tile,
+                                            long long istride) {
+  const int h = lane >> 3, j = lane & 7, t = threadIdx.x;
+  double2* wl = lds + wv * kRegion;
+  double2 nxt[kHB];
+  tile_load(nxt, tile, istride, 0, t);
 #pragma unroll
   for (int b0 = 0; b0 < kM; b0 += kHB) {
+    __syncthreads();  // every wave is done with the previous batch's regions
+    tile_to_lds(nxt, lds, t);
+    if (b0 + kHB < kM) tile_load(nxt, tile, istride, b0 + kHB, t);
+    __syncthreads();
+    double2 u[8];
 #pragma unroll
-    for (int n2 = 0; n2 < 8; ++n2) u[n2] = nxt[n2];
-    if (b0 + kHB < kM) z64_load_batch(nxt, col, istride, b0 + kHB, h, j, ok);
+    for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
     Dft<8, -1>::run(u);
 #pragma unroll
     for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<-1>(u[k2], tw[k2]);
+    wave_sync();
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
     wave_sync();
-    // lane (h, k2 = j): DFT over the first index
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
     Dft<8, -1>::run(u);
@@ -129,16 +143,18 @@ __device__ __forceinline__ void z64_forward(double2 (&v)[kM], double2* wl, int l
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (b0 + q < kM) v[b0 + q] = wl[q * kPadBatch + pad8(lane)];
-    wave_sync();
   }
 }
 
-__device__ __forceinline__ void z64_inverse(const double2 (&v)[kM], double2* wl, int lane,
-                                            const double2 (&tw)[8], double2* __restrict__ col,
-                                            long long istride, bool ok) {
-  const int h = lane >> 3, j = lane & 7;
+__device__ __forceinline__ void z64_inverse(const double2 (&v)[kM], double2* lds, int wv, int lane,
+                                            const double2 (&tw)[8], double2* __restrict__ tile,
+                                            long long istride) {
+  const int h = lane >> 3, j = lane & 7, t = threadIdx.x;
+  const int tz = t >> 2, tw4 = t & 3;
+  double2* wl = lds + wv * kRegion;
 #pragma unroll
   for (int b0 = 0; b0 < kM; b0 += kHB) {
+    __syncthreads();  // the previous batch's tiles have been read out of LDS
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (b0 + q < kM) wl[q * kPadBatch + pad8(lane)] = v[b0 + q];
@@ -156,43 +172,43 @@ __device__ __forceinline__ void z64_inverse(const double2 (&v)[kM], double2* wl,
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
     Dft<8, +1>::run(u);
-    if (ok && b0 + h < kM) {
-      double2* dst = col + (long long)(b0 + h) * istride + j;
-#pragma unroll
-      for (int k1 = 0; k1 < 8; ++k1) dst[8 * k1] = u[k1];
-    }
     wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < kM)
+        tile[(long long)(b0 + q) * istride + t] = lds[tw4 * kRegion + q * kPadBatch + pad8(tz)];
   }
 }
 
 template <int OCC>
 __global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
-  __shared__ double2 lds[4 * kHB * kPadBatch];
+  __shared__ double2 lds[4 * kRegion];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ny = g.ny, nxh = g.nxh;
-  const int col = blockIdx.x * 4 + wv;  // linear column index, kx fastest
-  const bool valid = col < nxh * ny;
-  const int ky = valid ? col / nxh : 0;
-  const int kx = valid ? col - ky * nxh : 0;
-  const long long istride = (long long)ny * nxh * 64;
-  double2* cp = a.spec + (long long)(valid ? col : 0) * 64;
-  // step-1 twiddles exp(-2 pi i j k2 / 64), j = lane & 7
+  const int t4 = blockIdx.x, ky = blockIdx.y;
+  const int kx = t4 * kKxTile + wv;
+  const bool valid = kx < nxh;
+  const long long istride = spec_index(1, 0, 0, 0, ny, g.nt, 64);
+  // the whole 4-kx tile is allocated even where kx >= nxh (the layout pads to 4)
+  double2* tile = a.spec + spec_index(0, ky, t4 * kKxTile, 0, ny, g.nt, 64);
   double2 tw[8];
   {
     const int j = lane & 7;
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) tw[k2] = g.az.tw[j * k2];
   }
-  double2* wl = lds + wv * kHB * kPadBatch;
   double2 v[kM];
-  z64_forward(v, wl, lane, tw, cp, istride, valid);
+  z64_forward(v, lds, wv, lane, tw, tile, istride);
   if (a.solve && valid) {
     const double kxv = signed_k(kx, g.nx) * a.kfac[0];
     const double kyv = signed_k(ky, ny) * a.kfac[1];
     const double kzv = signed_k(lane, 64) * a.kfac[2];
     Solver<kL>::solve(v, kxv, kyv, kzv, a.cdelta, a.sigma, a.ds, a.dslam);
   }
-  z64_inverse(v, wl, lane, tw, cp, istride, valid);
+  z64_inverse(v, lds, wv, lane, tw, tile, istride);
 }
 
 }  // namespace per_l_<L>
@@ -202,9 +218,9 @@ using namespace WLC_CAT(per_l_, WLC_L);
 hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hipStream_t st) {
   if (a.M != kM) return hipErrorInvalidValue;
   if (g.nz == 64 && !a.generic) {
-    const int ncol = g.nxh * g.ny;
-    if (a.occ == 2) hipLaunchKernelGGL(k_zsolve64<2>, dim3((ncol + 3) / 4), dim3(256), 0, st, g, a);
-    else hipLaunchKernelGGL(k_zsolve64<1>, dim3((ncol + 3) / 4), dim3(256), 0, st, g, a);
+    static_assert(kKxTile == 4, "one wave per kx of a tile");
+    if (a.occ == 2) hipLaunchKernelGGL(k_zsolve64<2>, dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
+    else hipLaunchKernelGGL(k_zsolve64<1>, dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (a.TX * g.nz > 256 || a.TX < 1) return hipErrorInvalidValue;
