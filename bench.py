@@ -24,7 +24,7 @@ sys.path.insert(0, os.path.join(HERE, "wormlike-chain-triblock_amd"))
 
 METRIC = "MDE contour-steps/s at 64^3 Lmax=6"
 PEAK_HBM_GBS = 8000.0
-KERNELS = ["rhs_fft_xy", "zsolve", "ifft_xy", "density_acc"]
+KERNELS = ["rhs_fft_xy", "zsolve", "ifft_xy", "density_acc", "inv_rhs_fft_xy"]
 
 
 def workload_w(shape=(64, 64, 64), seed=1, sigma=2.0, std=5.0, nblk=3):
@@ -52,6 +52,9 @@ def algorithmic_bytes(M, N, Nk):
         "zsolve": 2 * M * Nk * 16,
         "ifft_xy": M * Nk * 16 + M * N * 8,
         "density_acc": 2 * M * N * 8 + 2 * N * 8,
+        # inverse xy of step s fused with the rhs + forward xy of step s+1: q(s) stays on
+        # chip, so only two history slices are read
+        "inv_rhs_fft_xy": M * Nk * 16 + M * N * 8 + 2 * M * N * 8 + N * 8 + M * Nk * 16,
         "step": 4 * M * N * 8 + 8 * N + 64 * M * Nk,
     }
 

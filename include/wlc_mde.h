@@ -106,7 +106,8 @@ void* wlc_mde_stream(wlc_handle* h);
 /* Per-kernel timing with HIP events on the handle's stream.  While enabled, every
  * launch of the step kernels is bracketed by events; wlc_mde_get_timing returns,
  * for kernel k in {0: rhs+xy FFT, 1: z FFT+solve, 2: inverse xy FFT, 3: density
- * accumulation}, the summed milliseconds and the launch count since enabling.
+ * accumulation, 4: inverse xy FFT of one step fused with rhs+xy FFT of the next}, the
+ * summed milliseconds and the launch count since enabling.
  * No counterpart in the reference (which times only the whole run, SCFT_main.f90:51-55). */
 int wlc_mde_set_timing(wlc_handle* h, int enable);
 int wlc_mde_get_timing(wlc_handle* h, double* ms, long long* count, int nkernels);

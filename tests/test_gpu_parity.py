@@ -129,3 +129,39 @@ def test_set_box_changes_the_solution(wlc, oracle):
     assert abs(Q - oQ) / oQ < TOL
     assert _rel(phi, ophi) < TOL
     g.close()
+
+
+FUSE_CASES = [
+    # compile-time 32x32 planes, basis grows then shrinks at the junctions
+    ((8, 32, 32), (3.0, 3.0, 2.0), (5, 6, 4), (4, 6, 3), (1.0, 0.5, 2.0)),
+    # compile-time 64x64 planes, shrinking then growing basis, generic z kernel
+    ((6, 64, 64), (5.0, 5.0, 2.0), (4, 3, 5), (6, 2, 5), (1.0, 1.0, 1.0)),
+    # runtime plane sizes
+    ((12, 6, 18), (2.0, 2.5, 3.0), (3, 4, 4), (2, 3, 1), (0.8, 1.2, 0.6)),
+]
+
+
+@pytest.mark.parametrize("case", FUSE_CASES, ids=[f"{c[0]}-L{c[3]}" for c in FUSE_CASES])
+def test_fused_inverse_rhs_is_bit_identical(wlc, case, monkeypatch):
+    """The inverse xy pass of step s fused with the right-hand side of step s+1 keeps
+    q(s) on chip instead of reloading it: every slice must equal the unfused sweep's."""
+    shape, box, Nblk, L, lam = case
+    rng = np.random.default_rng(77)
+    w = 2.0 * rng.standard_normal((len(Nblk),) + shape)
+    res = []
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("WLC_FUSE", fuse)
+        g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, keep_qstar=True)
+        g.set_w(w)
+        g.MDE_q()
+        g.MDE_qstar()
+        phi, Q, td = g.density()
+        Ns = sum(Nblk)
+        res.append(([g.get_q(+1, s) for s in range(Ns + 1)],
+                    [g.get_q(-1, s) for s in range(Ns + 1)], phi, Q, td))
+        g.close()
+    (q0, qs0, phi0, Q0, td0), (q1, qs1, phi1, Q1, td1) = res
+    for s in range(len(q0)):
+        assert np.array_equal(q0[s], q1[s]), f"forward slice {s}"
+        assert np.array_equal(qs0[s], qs1[s]), f"backward slice {s}"
+    assert np.array_equal(phi0, phi1) and Q0 == Q1 and td0 == td1

@@ -38,7 +38,6 @@ struct RhsArgs {
   double ds;
   const double* w;     // field of the target block [nz][ny][nx]
   int M;               // harmonics of the target slice
-  int hpw;             // harmonics per workgroup (the w plane is read once per group)
   double2* spec;       // half spectrum, layout of spec_index
 };
 
@@ -64,6 +63,11 @@ struct InvArgs {
 hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_t st);
 hipError_t launch_zsolve(const Geom& g, const SolveArgs& a, hipStream_t st);
 hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st);
+// C of one step fused with A of the next: inverts and stores c.M harmonics of the
+// solved step, whose q stays on chip as history slice 0 of `a` (a.h[0] is not read),
+// and writes the next step's half spectrum for a.M harmonics
+hipError_t launch_inv_rhs(const Geom& g, const InvArgs& c, const RhsArgs& a, int order,
+                          hipStream_t st);
 
 // small elementwise/reduction kernels
 hipError_t launch_set_delta(const Geom& g, double* slice, int M, hipStream_t st);

@@ -94,14 +94,14 @@ struct wlc_handle {
   bool have_w = false, fwd_done = false, bwd_done = false;
   int generic_zsolve = 0;
   int zs_occ = 1;
-  int hpw = 4;
+  int fuse = 1;  // kernel C of a step fused with kernel A of the next
   // timing
   bool timing = false;
   struct Rec { int k; hipEvent_t a, b; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
-  double ms[4] = {0, 0, 0, 0};
-  long long cnt[4] = {0, 0, 0, 0};
+  double ms[5] = {0, 0, 0, 0, 0};
+  long long cnt[5] = {0, 0, 0, 0, 0};
   hipEvent_t uev[8] = {};
 };
 
@@ -184,51 +184,68 @@ hipError_t copy_fields_out(wlc_handle* h, double* dst, const double* src, int nf
   return hipSuccess;
 }
 
-int run_step(wlc_handle* h, int dir, int s) {
+// Right-hand-side arguments of the step whose target is s.  BDF order by distance from
+// the chain end the sweep starts at (wormlike_MDE_2dcomp_mod.f90:325-343 BDF1, :484-502
+// BDF2, :584-608 BDF3; backward :1046-1064, :1138-1157, :1234-1259); the field is the
+// target block's; history j is slice s - dir (j + 1) with its own harmonic count.
+void rhs_args(wlc_handle* h, int dir, int s, wlc::RhsArgs* ra, int* order, double* cdelta) {
+  const int Ns = h->Ns;
+  *order = dir > 0 ? std::min(s, 3) : std::min(Ns - s, 3);
+  const int blk = block_of(h, s);
+  for (int j = 0; j < 3; ++j) {
+    const int sh = s - dir * (j + 1);
+    if (j < *order && sh >= 0 && sh <= Ns) {
+      ra->h[j] = dir > 0 ? h->q + h->off[sh] : qs_slot(h, sh);
+      ra->hM[j] = h->Mb[block_of(h, sh)];
+    } else {
+      ra->h[j] = nullptr;
+      ra->hM[j] = 0;
+    }
+  }
+  if (*order == 1) {
+    ra->a[0] = 1.0; ra->a[1] = 0.0; ra->a[2] = 0.0;
+    ra->b[0] = 1.0; ra->b[1] = 0.0; ra->b[2] = 0.0;
+    *cdelta = 1.0;
+  } else if (*order == 2) {
+    ra->a[0] = 2.0; ra->a[1] = -0.5; ra->a[2] = 0.0;
+    ra->b[0] = 2.0; ra->b[1] = -1.0; ra->b[2] = 0.0;
+    *cdelta = 1.5;
+  } else {
+    ra->a[0] = 3.0; ra->a[1] = -1.5; ra->a[2] = 1.0 / 3.0;
+    ra->b[0] = 3.0; ra->b[1] = -3.0; ra->b[2] = 1.0;
+    *cdelta = 11.0 / 6.0;
+  }
+  ra->ds = h->ds;
+  ra->w = h->w + (long long)blk * h->hs;
+  ra->M = h->Mb[blk];
+  ra->spec = h->spec;
+}
+
+double* slice_of(wlc_handle* h, int dir, int s) { return dir > 0 ? h->q + h->off[s] : qs_slot(h, s); }
+
+// kernel A for target s (first step of a sweep, or every step when unfused)
+int step_rhs(wlc_handle* h, int dir, int s) {
+  wlc::RhsArgs ra{};
+  int order;
+  double cdelta;
+  rhs_args(h, dir, s, &ra, &order, &cdelta);
+  TimedLaunch t(h, 0);
+  HIPCHK(wlc::launch_rhs_fft(h->geom, ra, order, h->st));
+  return WLC_OK;
+}
+
+// kernel B for target s: z pass, solve of G^{c,sigma}(k) (Gmatrix_2dcomp_mod.f90:475-479),
+// inverse z pass
+int step_solve(wlc_handle* h, int dir, int s) {
   const int Ns = h->Ns;
   const int order = dir > 0 ? std::min(s, 3) : std::min(Ns - s, 3);
   const int blk = block_of(h, s);
-  const int M = h->Mb[blk];
-  wlc::RhsArgs ra{};
-  for (int j = 0; j < 3; ++j) {
-    const int sh = s - dir * (j + 1);
-    if (j < order && sh >= 0 && sh <= Ns) {
-      ra.h[j] = dir > 0 ? h->q + h->off[sh] : qs_slot(h, sh);
-      ra.hM[j] = h->Mb[block_of(h, sh)];
-    } else {
-      ra.h[j] = nullptr;
-      ra.hM[j] = 0;
-    }
-  }
-  double cdelta;
-  if (order == 1) {
-    ra.a[0] = 1.0; ra.a[1] = 0.0; ra.a[2] = 0.0;
-    ra.b[0] = 1.0; ra.b[1] = 0.0; ra.b[2] = 0.0;
-    cdelta = 1.0;
-  } else if (order == 2) {
-    ra.a[0] = 2.0; ra.a[1] = -0.5; ra.a[2] = 0.0;
-    ra.b[0] = 2.0; ra.b[1] = -1.0; ra.b[2] = 0.0;
-    cdelta = 1.5;
-  } else {
-    ra.a[0] = 3.0; ra.a[1] = -1.5; ra.a[2] = 1.0 / 3.0;
-    ra.b[0] = 3.0; ra.b[1] = -3.0; ra.b[2] = 1.0;
-    cdelta = 11.0 / 6.0;
-  }
-  ra.ds = h->ds;
-  ra.w = h->w + (long long)blk * h->hs;
-  ra.M = M;
-  ra.hpw = h->hpw;
-  ra.spec = h->spec;
-  {
-    TimedLaunch t(h, 0);
-    HIPCHK(wlc::launch_rhs_fft(h->geom, ra, order, h->st));
-  }
   wlc::SolveArgs sa{};
   sa.spec = h->spec;
-  sa.M = M;
+  sa.M = h->Mb[blk];
   sa.L = h->Lb[blk];
   sa.TX = h->TX[blk];
-  sa.cdelta = cdelta;
+  sa.cdelta = order == 1 ? 1.0 : (order == 2 ? 1.5 : 11.0 / 6.0);
   sa.sigma = dir > 0 ? 1.0 : -1.0;
   sa.ds = h->ds;
   sa.dslam = h->ds * h->p.lambda[blk];
@@ -237,18 +254,51 @@ int run_step(wlc_handle* h, int dir, int s) {
   sa.solve = 1;
   sa.generic = h->generic_zsolve;
   sa.occ = h->zs_occ;
-  {
-    TimedLaunch t(h, 1);
-    HIPCHK(wlc::launch_zsolve(h->geom, sa, h->st));
-  }
+  TimedLaunch t(h, 1);
+  HIPCHK(wlc::launch_zsolve(h->geom, sa, h->st));
+  return WLC_OK;
+}
+
+// kernel C for target s (inverse xy, 1/N, store), fused with kernel A of the next
+// target s + dir when `next` is set
+int step_inverse(wlc_handle* h, int dir, int s, bool next) {
   wlc::InvArgs ia{};
   ia.spec = h->spec;
-  ia.M = M;
+  ia.M = h->Mb[block_of(h, s)];
   ia.scale = 1.0 / (double)h->N;
-  ia.out = dir > 0 ? h->q + h->off[s] : qs_slot(h, s);
-  {
+  ia.out = slice_of(h, dir, s);
+  if (!next) {
     TimedLaunch t(h, 2);
     HIPCHK(wlc::launch_ifft_store(h->geom, ia, h->st));
+    return WLC_OK;
+  }
+  wlc::RhsArgs ra{};
+  int order;
+  double cdelta;
+  rhs_args(h, dir, s + dir, &ra, &order, &cdelta);
+  TimedLaunch t(h, 4);
+  HIPCHK(wlc::launch_inv_rhs(h->geom, ia, ra, order, h->st));
+  return WLC_OK;
+}
+
+int density_at(wlc_handle* h, int s);
+
+// the steps of one sweep, targets first .. last (inclusive, in direction dir); after
+// each target the optional density accumulation
+int sweep(wlc_handle* h, int dir, int first, int last, bool dens) {
+  int rc = step_rhs(h, dir, first);
+  if (rc) return rc;
+  for (int s = first;; s += dir) {
+    const bool more = s != last;
+    if ((rc = step_solve(h, dir, s))) return rc;
+    if (h->fuse) {
+      if ((rc = step_inverse(h, dir, s, more))) return rc;
+    } else {
+      if ((rc = step_inverse(h, dir, s, false))) return rc;
+    }
+    if (dens && (rc = density_at(h, s))) return rc;
+    if (!more) break;
+    if (!h->fuse && (rc = step_rhs(h, dir, s + dir))) return rc;
   }
   return WLC_OK;
 }
@@ -349,7 +399,7 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   h->p = *p;
   if (const char* ge = std::getenv("WLC_GENERIC_ZSOLVE")) h->generic_zsolve = std::atoi(ge) != 0;
   if (const char* oc = std::getenv("WLC_ZSOLVE_OCC")) h->zs_occ = std::atoi(oc) == 2 ? 2 : 1;
-  if (const char* hp = std::getenv("WLC_HPW")) h->hpw = std::max(1, std::atoi(hp));
+  if (const char* fu = std::getenv("WLC_FUSE")) h->fuse = std::atoi(fu) != 0;
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
     return fail(WLC_EHIP, "no HIP device");
@@ -474,10 +524,8 @@ int wlc_mde_forward(wlc_handle* h) {
   if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede forward");
   HIPCHK(hipSetDevice(h->device));
   HIPCHK(wlc::launch_set_delta(h->geom, h->q, h->Mb[block_of(h, 0)], h->st));
-  for (int s = 1; s <= h->Ns; ++s) {
-    int rc = run_step(h, +1, s);
-    if (rc) return rc;
-  }
+  int rc = sweep(h, +1, 1, h->Ns, false);
+  if (rc) return rc;
   h->fwd_done = true;
   h->bwd_done = false;
   if (h->timing) return flush_timing(h);
@@ -496,14 +544,8 @@ int wlc_mde_backward(wlc_handle* h) {
     int rc = density_at(h, Ns);
     if (rc) return rc;
   }
-  for (int s = Ns - 1; s >= 0; --s) {
-    int rc = run_step(h, -1, s);
-    if (rc) return rc;
-    if (dens) {
-      rc = density_at(h, s);
-      if (rc) return rc;
-    }
-  }
+  int rc = sweep(h, -1, Ns - 1, 0, dens);
+  if (rc) return rc;
   h->bwd_done = dens;
   if (h->timing) return flush_timing(h);
   return WLC_OK;
@@ -590,7 +632,7 @@ int wlc_mde_set_timing(wlc_handle* h, int enable) {
   int rc = flush_timing(h);
   if (rc) return rc;
   h->timing = enable != 0;
-  for (int k = 0; k < 4; ++k) { h->ms[k] = 0.0; h->cnt[k] = 0; }
+  for (int k = 0; k < 5; ++k) { h->ms[k] = 0.0; h->cnt[k] = 0; }
   return WLC_OK;
 }
 
@@ -598,7 +640,7 @@ int wlc_mde_get_timing(wlc_handle* h, double* ms, long long* count, int nk) {
   if (!h) return fail(WLC_EINVAL, "NULL handle");
   int rc = flush_timing(h);
   if (rc) return rc;
-  for (int k = 0; k < nk && k < 4; ++k) {
+  for (int k = 0; k < nk && k < 5; ++k) {
     if (ms) ms[k] = h->ms[k];
     if (count) count[k] = h->cnt[k];
   }

@@ -58,111 +58,69 @@ __device__ __forceinline__ void plane_ifft_x(double2* lds, const Geom& g) {
 template <int NX>
 __device__ __forceinline__ int lp(int u) { return NX > 0 ? lds_pad(u) : u; }
 
-// A: per (z-plane, group of hpw harmonics).  The w plane is read once per workgroup.
-template <int NX, int NY, int MAXE, int ORDER>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1))) k_rhs_fft_xy(Geom g, RhsArgs a) {
-  extern __shared__ double2 lds[];
-  int z, grp;
-  plane_of_block(g.nz, z, grp);
+// ---------------------------------------------------------------------------
+// plane stages shared by kernels A, C and the fused C+A kernel
+// ---------------------------------------------------------------------------
+// The real plane lives in LDS as packed rows: row pair (2j, 2j+1) is the complex row
+// j, value (x, y) at double index 2 lp(u) + (y & 1) with u = (y >> 1) nx + x.
+template <int NX>
+__device__ __forceinline__ int pk_at(int y, int x, int nx) {
+  return 2 * lp<NX>((y >> 1) * nx + x) + (y & 1);
+}
+
+// packed real plane in LDS -> x pass (two real rows per complex transform), unpack,
+// y pass -> half-spectrum plane (i, z)
+template <int NX, int NY, int MAXE>
+__device__ __forceinline__ void forward_xy(double2* lds, const Geom& g, double2* __restrict__ spec,
+                                          int i, int z) {
   const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
   const int nxh = nx / 2 + 1;
-  const int nz = g.nz;
-  const long long hs = g.hs;
-  const long long plane = (long long)z * g.zs;
-  const int tot = nx * ny;
-  const double* w = a.w + plane;
-  double* pk = reinterpret_cast<double*>(lds);  // packed rows: (row 2j, row 2j+1) -> complex
-  const double ds = a.ds;
-  const double a0 = a.a[0], a1 = a.a[1], a2 = a.a[2];
-  const double b0 = a.b[0], b1 = a.b[1], b2 = a.b[2];
-  const int i0 = grp * a.hpw;
-  const int i1 = min(i0 + a.hpw, a.M);
-  const double2* w2 = reinterpret_cast<const double2*>(w);
-  const int tot2 = tot >> 1;  // nx is even: x pairs (2p, 2p+1) share a row
-  for (int i = i0; i < i1; ++i) {
-    const bool u1 = i < a.hM[0];
-    const bool u2 = ORDER >= 2 && i < a.hM[1];
-    const bool u3 = ORDER >= 3 && i < a.hM[2];
-    const double2* q1 = reinterpret_cast<const double2*>(u1 ? a.h[0] + i * hs + plane : w);
-    const double2* q2 = reinterpret_cast<const double2*>(u2 ? a.h[1] + i * hs + plane : w);
-    const double2* q3 = reinterpret_cast<const double2*>(u3 ? a.h[2] + i * hs + plane : w);
-    for (int base = 0; base < tot2; base += kPairs * kThreads) {
-      double2 v1[kPairs], v2[kPairs], v3[kPairs], ww[kPairs];
-      const double2 zero = make_double2(0.0, 0.0);
+  __syncthreads();
+  plane_fft_x<NX, NY, MAXE>(lds, g);
+  // unpack Z = A + iB into A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i)
+  {
+    constexpr int UPT = NX > 0 ? ((NY / 2) * (NX / 2 + 1) + kThreads - 1) / kThreads
+                               : (MAXE + 1) / 2 + 1;
+    double2 ra[UPT], rb[UPT];
+    int dst[UPT];
+    const int tu = (ny / 2) * nxh;
 #pragma unroll
-      for (int c = 0; c < kPairs; ++c) {
-        const int e = base + c * kThreads + threadIdx.x;
-        const bool in = e < tot2;
-        v1[c] = (u1 && in) ? q1[e] : zero;
-        v2[c] = (u2 && in) ? q2[e] : zero;
-        v3[c] = (u3 && in) ? q3[e] : zero;
-        ww[c] = in ? w2[e] : zero;
-      }
-#pragma unroll
-      for (int c = 0; c < kPairs; ++c) {
-        const int e = base + c * kThreads + threadIdx.x;
-        if (e < tot2) {
-          const int y = (2 * e) / nx, x = 2 * e - y * nx;
-          const double t0 = a0 * v1[c].x + a1 * v2[c].x + a2 * v3[c].x;
-          const double s0 = b0 * v1[c].x + b1 * v2[c].x + b2 * v3[c].x;
-          const double t1 = a0 * v1[c].y + a1 * v2[c].y + a2 * v3[c].y;
-          const double s1 = b0 * v1[c].y + b1 * v2[c].y + b2 * v3[c].y;
-          const int u = (y >> 1) * nx + x;
-          pk[2 * lp<NX>(u) + (y & 1)] = t0 - (ds * ww[c].x) * s0;
-          pk[2 * lp<NX>(u + 1) + (y & 1)] = t1 - (ds * ww[c].y) * s1;
-        }
+    for (int c = 0; c < UPT; ++c) {
+      const int idx = threadIdx.x + c * kThreads;
+      dst[c] = -1;
+      if (idx < tu) {
+        const int j = idx / nxh, k = idx - j * nxh;
+        const double2 zk = lds[lp<NX>(j * nx + k)];
+        const double2 zn = lds[lp<NX>(j * nx + (k == 0 ? 0 : nx - k))];
+        ra[c] = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+        rb[c] = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
+        dst[c] = (2 * j) * nxh + k;
       }
     }
     __syncthreads();
-    plane_fft_x<NX, NY, MAXE>(lds, g);
-    // unpack Z = A + iB into A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i)
-    {
-      constexpr int UPT = NX > 0 ? ((NY / 2) * (NX / 2 + 1) + kThreads - 1) / kThreads
-                                 : (MAXE + 1) / 2 + 1;
-      double2 ra[UPT], rb[UPT];
-      int dst[UPT];
-      const int tu = (ny / 2) * nxh;
 #pragma unroll
-      for (int c = 0; c < UPT; ++c) {
-        const int idx = threadIdx.x + c * kThreads;
-        dst[c] = -1;
-        if (idx < tu) {
-          const int j = idx / nxh, k = idx - j * nxh;
-          const double2 zk = lds[lp<NX>(j * nx + k)];
-          const double2 zn = lds[lp<NX>(j * nx + (k == 0 ? 0 : nx - k))];
-          ra[c] = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
-          rb[c] = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
-          dst[c] = (2 * j) * nxh + k;
-        }
+    for (int c = 0; c < UPT; ++c)
+      if (dst[c] >= 0) {
+        lds[lp<NX>(dst[c])] = ra[c];
+        lds[lp<NX>(dst[c] + nxh)] = rb[c];
       }
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < UPT; ++c)
-        if (dst[c] >= 0) {
-          lds[lp<NX>(dst[c])] = ra[c];
-          lds[lp<NX>(dst[c] + nxh)] = rb[c];
-        }
-      __syncthreads();
-    }
-    plane_fft_y<NX, NY, MAXE, -1>(lds, g);
-    // half spectrum, 64-byte runs of 4 kx per (ky, tile, z)
-    for (int e = threadIdx.x; e < ny * nxh; e += kThreads) {
-      const int ky = e / nxh, kx = e - ky * nxh;
-      a.spec[spec_index(i, ky, kx, z, ny, g.nt, nz)] = lds[lp<NX>(e)];
-    }
     __syncthreads();
+  }
+  plane_fft_y<NX, NY, MAXE, -1>(lds, g);
+  // half spectrum, 64-byte runs of 4 kx per (ky, tile, z)
+  for (int e = threadIdx.x; e < ny * nxh; e += kThreads) {
+    const int ky = e / nxh, kx = e - ky * nxh;
+    spec[spec_index(i, ky, kx, z, ny, g.nt, g.nz)] = lds[lp<NX>(e)];
   }
 }
 
-// C: per (z-plane, harmonic)
+// half-spectrum plane (i, z) -> y pass, Hermitian pack, x pass -> packed real plane
+// in LDS (unscaled)
 template <int NX, int NY, int MAXE>
-__global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
-  extern __shared__ double2 lds[];
-  int z, i;
-  plane_of_block(g.nz, z, i);
+__device__ __forceinline__ void inverse_xy(double2* lds, const Geom& g,
+                                          const double2* __restrict__ spec, int i, int z) {
   const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
   const int nxh = nx / 2 + 1;
-  const int nz = g.nz;
   const int tin = ny * nxh;
   for (int base = 0; base < tin; base += kChunk * kThreads) {
     double2 r[kChunk];
@@ -170,7 +128,7 @@ __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
     for (int c = 0; c < kChunk; ++c) {
       const int e = base + c * kThreads + threadIdx.x;
       const int ky = e / nxh, kx = e - ky * nxh;
-      r[c] = e < tin ? a.spec[spec_index(i, ky, kx, z, ny, g.nt, nz)] : make_double2(0.0, 0.0);
+      r[c] = e < tin ? spec[spec_index(i, ky, kx, z, ny, g.nt, g.nz)] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int c = 0; c < kChunk; ++c) {
@@ -219,13 +177,122 @@ __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
     __syncthreads();
   }
   plane_ifft_x<NX, NY, MAXE>(lds, g);
+}
+
+// One point of the BDF right-hand side with a fixed rounding sequence (explicit fma, so
+// that every kernel forming it, fused or not, produces the same bits).
+__device__ __forceinline__ double bdf_rhs(double q1, double q2, double q3, double w, double a0,
+                                          double a1, double a2, double b0, double b1, double b2,
+                                          double ds) {
+  const double t = __builtin_fma(a2, q3, __builtin_fma(a1, q2, a0 * q1));
+  const double u = __builtin_fma(b2, q3, __builtin_fma(b1, q2, b0 * q1));
+  return __builtin_fma(-(ds * w), u, t);
+}
+
+// BDF right-hand side of harmonic i on z-plane `plane` into the packed LDS plane:
+//   P = a0 q1 + a1 q2 + a2 q3 - ds w (b0 q1 + b1 q2 + b2 q3)
+// q1 is read from history slice 0, or (FROM_LDS) taken from the packed LDS plane times
+// `scale` (the freshly inverted q(s), also stored to `out` when `store`).  have1 = 0
+// makes q1 zero; rhs = 0 skips the right-hand side (store only).
+template <int NX, int NY, int ORDER, bool FROM_LDS>
+__device__ __forceinline__ void rhs_to_lds(double* pk, const Geom& g, const RhsArgs& a, int i,
+                                           long long plane, bool have1, double scale,
+                                           double* __restrict__ out, bool store, bool rhs) {
+  const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
+  const long long hs = g.hs;
+  const int tot2 = (nx * ny) >> 1;  // nx is even: x pairs (2p, 2p+1) share a row
+  const double ds = a.ds;
+  const double a0 = a.a[0], a1 = a.a[1], a2 = a.a[2];
+  const double b0 = a.b[0], b1 = a.b[1], b2 = a.b[2];
+  const bool u1 = have1 && (FROM_LDS || i < a.hM[0]);
+  const bool u2 = rhs && ORDER >= 2 && i < a.hM[1];
+  const bool u3 = rhs && ORDER >= 3 && i < a.hM[2];
+  const double2* w2 = reinterpret_cast<const double2*>(a.w + plane);
+  const double2* q1 =
+      reinterpret_cast<const double2*>(u1 && !FROM_LDS ? a.h[0] + i * hs + plane : a.w + plane);
+  const double2* q2 = reinterpret_cast<const double2*>(u2 ? a.h[1] + i * hs + plane : a.w + plane);
+  const double2* q3 = reinterpret_cast<const double2*>(u3 ? a.h[2] + i * hs + plane : a.w + plane);
+  double2* o2 = reinterpret_cast<double2*>(out);
+  const double2 zero = make_double2(0.0, 0.0);
+  for (int base = 0; base < tot2; base += kPairs * kThreads) {
+    double2 v1[kPairs], v2[kPairs], v3[kPairs], ww[kPairs];
+#pragma unroll
+    for (int c = 0; c < kPairs; ++c) {
+      const int e = base + c * kThreads + threadIdx.x;
+      const bool in = e < tot2;
+      v1[c] = (!FROM_LDS && u1 && in) ? q1[e] : zero;
+      v2[c] = (u2 && in) ? q2[e] : zero;
+      v3[c] = (u3 && in) ? q3[e] : zero;
+      ww[c] = (rhs && in) ? w2[e] : zero;
+    }
+#pragma unroll
+    for (int c = 0; c < kPairs; ++c) {
+      const int e = base + c * kThreads + threadIdx.x;
+      if (e < tot2) {
+        const int y = (2 * e) / nx, x = 2 * e - y * nx;
+        const int p0 = pk_at<NX>(y, x, nx), p1 = pk_at<NX>(y, x + 1, nx);
+        if (FROM_LDS && u1) {
+          v1[c] = make_double2(pk[p0] * scale, pk[p1] * scale);
+          if (store) o2[e] = v1[c];
+        }
+        if (rhs) {
+          pk[p0] = bdf_rhs(v1[c].x, v2[c].x, v3[c].x, ww[c].x, a0, a1, a2, b0, b1, b2, ds);
+          pk[p1] = bdf_rhs(v1[c].y, v2[c].y, v3[c].y, ww[c].y, a0, a1, a2, b0, b1, b2, ds);
+        }
+      }
+    }
+  }
+}
+
+// A: per (z-plane, harmonic): BDF right-hand side from the history slices, forward
+// x and y passes
+template <int NX, int NY, int MAXE, int ORDER>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1)))
+k_rhs_fft_xy(Geom g, RhsArgs a) {
+  extern __shared__ double2 lds[];
+  int z, i;
+  plane_of_block(g.nz, z, i);
+  const long long plane = (long long)z * g.zs;
+  rhs_to_lds<NX, NY, ORDER, false>(reinterpret_cast<double*>(lds), g, a, i, plane, true, 1.0,
+                               nullptr, false, true);
+  forward_xy<NX, NY, MAXE>(lds, g, a.spec, i, z);
+}
+
+// C: per (z-plane, harmonic): inverse y and x (C2R) passes, 1/N, store of q(s)
+template <int NX, int NY, int MAXE>
+__global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
+  extern __shared__ double2 lds[];
+  int z, i;
+  plane_of_block(g.nz, z, i);
+  const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
+  inverse_xy<NX, NY, MAXE>(lds, g, a.spec, i, z);
   double* out = a.out + i * g.hs + (long long)z * g.zs;
   const double* pk = reinterpret_cast<const double*>(lds);
   const double sc = a.scale;
   for (int e = threadIdx.x; e < nx * ny; e += kThreads) {
     const int y = e / nx, x = e - y * nx;
-    out[e] = pk[2 * lp<NX>((y >> 1) * nx + x) + (y & 1)] * sc;
+    out[e] = pk[pk_at<NX>(y, x, nx)] * sc;
   }
+}
+
+// C of step s fused with A of the next step: per (z-plane, harmonic i <
+// max(M(s), M(next))).  For i < M(s) the solved plane is inverted and q_i(s) stored;
+// it stays in LDS as the newest history plane of the next right-hand side (zero for
+// i >= M(s): a junction that grows the basis).  For i < M(next) the right-hand side
+// is formed in place and transformed forward into the same spectrum plane.
+template <int NX, int NY, int MAXE, int ORDER>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1)))
+k_inv_rhs(Geom g, InvArgs c, RhsArgs a) {
+  extern __shared__ double2 lds[];
+  int z, i;
+  plane_of_block(g.nz, z, i);
+  const long long plane = (long long)z * g.zs;
+  const bool inv = i < c.M;
+  const bool rhs = i < a.M;
+  if (inv) inverse_xy<NX, NY, MAXE>(lds, g, c.spec, i, z);
+  rhs_to_lds<NX, NY, ORDER, true>(reinterpret_cast<double*>(lds), g, a, i, plane, inv, c.scale,
+                              c.out + i * g.hs + plane, inv, rhs);
+  if (rhs) forward_xy<NX, NY, MAXE>(lds, g, a.spec, i, z);
 }
 
 namespace {
@@ -249,6 +316,16 @@ static void* rhs_kernel(int order) {
   return nullptr;
 }
 
+template <int NX, int NY, int MAXE>
+static void* inv_rhs_kernel(int order) {
+  switch (order) {
+    case 1: return (void*)k_inv_rhs<NX, NY, MAXE, 1>;
+    case 2: return (void*)k_inv_rhs<NX, NY, MAXE, 2>;
+    case 3: return (void*)k_inv_rhs<NX, NY, MAXE, 3>;
+  }
+  return nullptr;
+}
+
 // square planes with compile-time sizes; everything else takes the runtime kernels
 static bool specialised(const Geom& g) {
   return g.nx == g.ny && (g.nx == 32 || g.nx == 64);
@@ -259,50 +336,51 @@ static size_t plane_lds_bytes(const Geom& g) {
   return sizeof(double2) * (specialised(g) ? n + n / 8 + 1 : n);
 }
 
-hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_t st) {
-  const size_t lds = plane_lds_bytes(g);
-  void* k = nullptr;
-  if (specialised(g)) {
-    k = g.nx == 32 ? rhs_kernel<32, 32, 0>(order) : rhs_kernel<64, 64, 0>(order);
-  } else {
-    switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
-      case 8: k = rhs_kernel<0, 0, 8>(order); break;
-      case 16: k = rhs_kernel<0, 0, 16>(order); break;
-      case 32: k = rhs_kernel<0, 0, 32>(order); break;
-      case 64: k = rhs_kernel<0, 0, 64>(order); break;
-      default: return hipErrorInvalidValue;
-    }
+// kind 0: A (order), 1: C, 2: fused C + A (order of the next step)
+static void* pick_plane_kernel(const Geom& g, int kind, int order) {
+#define WLC_PICK(NX, NY, MAXE)                                                     \
+  (kind == 0 ? rhs_kernel<NX, NY, MAXE>(order)                                     \
+             : kind == 1 ? (void*)k_ifft_xy<NX, NY, MAXE> : inv_rhs_kernel<NX, NY, MAXE>(order))
+  if (specialised(g)) return g.nx == 32 ? WLC_PICK(32, 32, 0) : WLC_PICK(64, 64, 0);
+  switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
+    case 8: return WLC_PICK(0, 0, 8);
+    case 16: return WLC_PICK(0, 0, 16);
+    case 32: return WLC_PICK(0, 0, 32);
+    case 64: return WLC_PICK(0, 0, 64);
   }
-  if (!k || a.hpw < 1) return hipErrorInvalidValue;
+#undef WLC_PICK
+  return nullptr;
+}
+
+static hipError_t launch_plane(const Geom& g, void* k, int planes, void** args, hipStream_t st) {
+  if (!k) return hipErrorInvalidValue;
+  const size_t lds = plane_lds_bytes(g);
   hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
+  return hipLaunchKernel(k, dim3(g.nz * planes), dim3(kThreads), args, lds, st);
+}
+
+hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_t st) {
   Geom gg = g;
   RhsArgs aa = a;
   void* args[] = {&gg, &aa};
-  const int groups = (a.M + a.hpw - 1) / a.hpw;
-  return hipLaunchKernel(k, dim3(g.nz * groups), dim3(kThreads), args, lds, st);
+  return launch_plane(g, pick_plane_kernel(g, 0, order), a.M, args, st);
 }
 
 hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st) {
-  const size_t lds = plane_lds_bytes(g);
-  void* k = nullptr;
-  if (specialised(g)) {
-    k = g.nx == 32 ? (void*)k_ifft_xy<32, 32, 0> : (void*)k_ifft_xy<64, 64, 0>;
-  } else {
-    switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
-      case 8: k = (void*)k_ifft_xy<0, 0, 8>; break;
-      case 16: k = (void*)k_ifft_xy<0, 0, 16>; break;
-      case 32: k = (void*)k_ifft_xy<0, 0, 32>; break;
-      case 64: k = (void*)k_ifft_xy<0, 0, 64>; break;
-      default: return hipErrorInvalidValue;
-    }
-  }
-  hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
   Geom gg = g;
   InvArgs aa = a;
   void* args[] = {&gg, &aa};
-  return hipLaunchKernel(k, dim3(g.nz * a.M), dim3(kThreads), args, lds, st);
+  return launch_plane(g, pick_plane_kernel(g, 1, 0), a.M, args, st);
+}
+
+hipError_t launch_inv_rhs(const Geom& g, const InvArgs& c, const RhsArgs& a, int order,
+                          hipStream_t st) {
+  Geom gg = g;
+  InvArgs cc = c;
+  RhsArgs aa = a;
+  void* args[] = {&gg, &cc, &aa};
+  return launch_plane(g, pick_plane_kernel(g, 2, order), c.M > a.M ? c.M : a.M, args, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -204,9 +204,9 @@ class WormlikeMDE:
         _check(_lib.wlc_mde_set_timing(self.h, 1 if enable else 0))
 
     def get_timing(self):
-        ms = (ctypes.c_double * 4)()
-        cnt = (ctypes.c_longlong * 4)()
-        _check(_lib.wlc_mde_get_timing(self.h, ms, cnt, 4))
+        ms = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_longlong * 5)()
+        _check(_lib.wlc_mde_get_timing(self.h, ms, cnt, 5))
         return list(ms), list(cnt)
 
     def event_record(self, slot):
