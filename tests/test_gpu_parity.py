@@ -44,6 +44,9 @@ CASES = [
     ((24, 4, 6), (2.0, 2.0, 5.0), (6, 7), (0, 2), (1.0, 0.0), 0),
     ((2, 2, 2), (1.0, 1.0, 1.0), (2, 1, 2), (8, 5, 6), (0.5, 1.0, 3.0), 0),
     ((16, 4, 4), (2.0, 2.0, 2.0), (3, 3, 3), (6, 6, 6), (1.0, 1.0, 1.0), 0),
+    # 64 x 64 planes (register-resident plane kernels), basis shrinks then grows
+    ((4, 64, 64), (2.0, 5.0, 4.0), (3, 4, 3), (3, 2, 3), (1.0, 0.5, 2.0), 0),
+    ((2, 64, 64), (1.0, 3.0, 3.0), (5,), (4,), (0.8,), 0),
 ]
 
 

@@ -326,22 +326,41 @@ static void* inv_rhs_kernel(int order) {
   return nullptr;
 }
 
-// square planes with compile-time sizes; everything else takes the runtime kernels
-static bool specialised(const Geom& g) {
-  return g.nx == g.ny && (g.nx == 32 || g.nx == 64);
-}
+// 32 x 32 planes take compile-time Stockham passes (64 x 64 planes have their own
+// kernels), everything else the runtime kernels
+static bool specialised(const Geom& g) { return g.nx == 32 && g.ny == 32; }
 
 static size_t plane_lds_bytes(const Geom& g) {
   const size_t n = (size_t)g.ny * g.nxh;
   return sizeof(double2) * (specialised(g) ? n + n / 8 + 1 : n);
 }
 
+}  // namespace wlc
+#include "wlc_plane64.h"
+namespace wlc {
+
+template <int ORDER>
+static void* p64_kernel(int kind) {
+  return kind == 0 ? (void*)p64::k_rhs64<ORDER> : (void*)p64::k_inv_rhs64<ORDER>;
+}
+
 // kind 0: A (order), 1: C, 2: fused C + A (order of the next step)
-static void* pick_plane_kernel(const Geom& g, int kind, int order) {
+static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds) {
+  *lds = plane_lds_bytes(g);
+  if (g.nx == 64 && g.ny == 64) {  // static 32 KB plane
+    *lds = 0;
+    if (kind == 1) return (void*)p64::k_inv64;
+    switch (order) {
+      case 1: return p64_kernel<1>(kind);
+      case 2: return p64_kernel<2>(kind);
+      case 3: return p64_kernel<3>(kind);
+    }
+    return nullptr;
+  }
 #define WLC_PICK(NX, NY, MAXE)                                                     \
   (kind == 0 ? rhs_kernel<NX, NY, MAXE>(order)                                     \
              : kind == 1 ? (void*)k_ifft_xy<NX, NY, MAXE> : inv_rhs_kernel<NX, NY, MAXE>(order))
-  if (specialised(g)) return g.nx == 32 ? WLC_PICK(32, 32, 0) : WLC_PICK(64, 64, 0);
+  if (specialised(g)) return WLC_PICK(32, 32, 0);
   switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
     case 8: return WLC_PICK(0, 0, 8);
     case 16: return WLC_PICK(0, 0, 16);
@@ -352,11 +371,15 @@ static void* pick_plane_kernel(const Geom& g, int kind, int order) {
   return nullptr;
 }
 
-static hipError_t launch_plane(const Geom& g, void* k, int planes, void** args, hipStream_t st) {
+static hipError_t launch_plane(const Geom& g, int kind, int order, int planes, void** args,
+                               hipStream_t st) {
+  size_t lds = 0;
+  void* k = pick_plane_kernel(g, kind, order, &lds);
   if (!k) return hipErrorInvalidValue;
-  const size_t lds = plane_lds_bytes(g);
-  hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
+  if (lds) {
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   return hipLaunchKernel(k, dim3(g.nz * planes), dim3(kThreads), args, lds, st);
 }
 
@@ -364,14 +387,14 @@ hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_
   Geom gg = g;
   RhsArgs aa = a;
   void* args[] = {&gg, &aa};
-  return launch_plane(g, pick_plane_kernel(g, 0, order), a.M, args, st);
+  return launch_plane(g, 0, order, a.M, args, st);
 }
 
 hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st) {
   Geom gg = g;
   InvArgs aa = a;
   void* args[] = {&gg, &aa};
-  return launch_plane(g, pick_plane_kernel(g, 1, 0), a.M, args, st);
+  return launch_plane(g, 1, 0, a.M, args, st);
 }
 
 hipError_t launch_inv_rhs(const Geom& g, const InvArgs& c, const RhsArgs& a, int order,
@@ -380,7 +403,7 @@ hipError_t launch_inv_rhs(const Geom& g, const InvArgs& c, const RhsArgs& a, int
   InvArgs cc = c;
   RhsArgs aa = a;
   void* args[] = {&gg, &cc, &aa};
-  return launch_plane(g, pick_plane_kernel(g, 2, order), c.M > a.M ? c.M : a.M, args, st);
+  return launch_plane(g, 2, order, c.M > a.M ? c.M : a.M, args, st);
 }
 
 // ---------------------------------------------------------------------------
