@@ -38,7 +38,7 @@ def workload_w(shape=(64, 64, 64), seed=1, sigma=2.0, std=5.0, nblk=3):
     kx = 2 * np.pi * np.fft.rfftfreq(nx)[None, None, :]
     filt = np.exp(-0.5 * sigma ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
     for b in range(nblk):
-        f = np.fft.irfftn(np.fft.rfftn(w[b]) * filt, s=shape)
+        f = np.fft.irfftn(np.fft.rfftn(w[b]) * filt, s=shape, axes=(0, 1, 2))
         f -= f.mean()
         w[b] = f * (std / f.std())
     return w
@@ -101,12 +101,26 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--traffic-file", default=os.path.join(HERE, "profiles", "traffic_64cube_L6.txt"))
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (the counter "
+                         "calibration kernel of wormlike-chain-triblock_amd/profile_summary.py "
+                         "traffic; run as a process of its own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the propagator solve runs on a single device "
               f"(multi-GPU sharding is a design note, not built); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_only:
+        import torch
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 3}))
+        return
 
     import wlc_mde as W
 

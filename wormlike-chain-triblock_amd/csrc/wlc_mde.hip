@@ -91,6 +91,14 @@ struct wlc_handle {
   wlc::AxisPlan plan[3];
   wlc::Geom geom{};
   hipStream_t st = nullptr;
+  // The backward sweep runs the density accumulation of slice s on st2, overlapped
+  // with the z solve of the next step on st: st2 waits for ev_step (slice s complete),
+  // and before the backward ring slot of slice s is rewritten (four steps later) st
+  // waits for ev_dens[s & 3].
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_step = nullptr;
+  hipEvent_t ev_dens[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool dens_used[4] = {false, false, false, false};
   bool have_w = false, fwd_done = false, bwd_done = false;
   int generic_zsolve = 0;
   int zs_occ = 1;
@@ -132,6 +140,7 @@ hipEvent_t take_event(wlc_handle* h) {
 int flush_timing(wlc_handle* h) {
   if (h->recs.empty()) return WLC_OK;
   HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipStreamSynchronize(h->st2));
   for (auto& r : h->recs) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
@@ -147,17 +156,18 @@ int flush_timing(wlc_handle* h) {
 struct TimedLaunch {
   wlc_handle* h;
   int k;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  TimedLaunch(wlc_handle* hh, int kk) : h(hh), k(kk) {
+  TimedLaunch(wlc_handle* hh, int kk, hipStream_t ss = nullptr) : h(hh), k(kk), s(ss ? ss : hh->st) {
     if (h->timing) {
       a = take_event(h);
       b = take_event(h);
-      if (a) (void)hipEventRecord(a, h->st);
+      if (a) (void)hipEventRecord(a, s);
     }
   }
   ~TimedLaunch() {
     if (h->timing && a && b) {
-      (void)hipEventRecord(b, h->st);
+      (void)hipEventRecord(b, s);
       h->recs.push_back({k, a, b});
     }
   }
@@ -262,6 +272,7 @@ int step_solve(wlc_handle* h, int dir, int s) {
 // kernel C for target s (inverse xy, 1/N, store), fused with kernel A of the next
 // target s + dir when `next` is set
 int step_inverse(wlc_handle* h, int dir, int s, bool next) {
+  if (dir < 0 && h->dens_used[s & 3]) HIPCHK(hipStreamWaitEvent(h->st, h->ev_dens[s & 3], 0));
   wlc::InvArgs ia{};
   ia.spec = h->spec;
   ia.M = h->Mb[block_of(h, s)];
@@ -318,9 +329,25 @@ int density_at(wlc_handle* h, int s) {
     ++n;
   }
   if (n == 0) return WLC_OK;
-  TimedLaunch t(h, 3);
-  HIPCHK(wlc::launch_density_acc(h->geom, h->q + h->off[s], qs_slot(h, s), rho[0], lim[0], wt[0],
-                                 rho[1], lim[1], wt[1], h->st));
+  HIPCHK(hipEventRecord(h->ev_step, h->st));
+  HIPCHK(hipStreamWaitEvent(h->st2, h->ev_step, 0));
+  {
+    TimedLaunch t(h, 3, h->st2);
+    HIPCHK(wlc::launch_density_acc(h->geom, h->q + h->off[s], qs_slot(h, s), rho[0], lim[0],
+                                   wt[0], rho[1], lim[1], wt[1], h->st2));
+  }
+  HIPCHK(hipEventRecord(h->ev_dens[s & 3], h->st2));
+  h->dens_used[s & 3] = true;
+  return WLC_OK;
+}
+
+// st waits for every pending density launch on st2
+int join_density(wlc_handle* h) {
+  for (int k = 0; k < 4; ++k)
+    if (h->dens_used[k]) {
+      HIPCHK(hipStreamWaitEvent(h->st, h->ev_dens[k], 0));
+      h->dens_used[k] = false;
+    }
   return WLC_OK;
 }
 
@@ -328,6 +355,7 @@ void free_handle(wlc_handle* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
+  if (h->st2) (void)hipStreamSynchronize(h->st2);
   for (auto& r : h->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);
   for (auto e : h->uev) if (e) (void)hipEventDestroy(e);
@@ -340,6 +368,9 @@ void free_handle(wlc_handle* h) {
   (void)hipFree(h->partial);
   (void)hipFree(h->scal);
   for (auto t : h->tw) (void)hipFree(t);
+  if (h->ev_step) (void)hipEventDestroy(h->ev_step);
+  for (auto e : h->ev_dens) if (e) (void)hipEventDestroy(e);
+  if (h->st2) (void)hipStreamDestroy(h->st2);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
 }
@@ -451,9 +482,16 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
       return fail(WLC_ENOMEM, std::string("device allocation failed: ") + #ptr); \
     }                                                                            \
   } while (0)
-  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking) != hipSuccess) {
     free_handle(h);
     return fail(WLC_EHIP, "stream creation failed");
+  }
+  bool ev_ok = hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) == hipSuccess;
+  for (auto& e : h->ev_dens) ev_ok = ev_ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  if (!ev_ok) {
+    free_handle(h);
+    return fail(WLC_EHIP, "event creation failed");
   }
   ALLOC(h->q, sizeof(double) * (size_t)h->off[h->Ns + 1]);
   if (p->keep_qstar) ALLOC(h->qs, sizeof(double) * (size_t)h->off[h->Ns + 1]);
@@ -545,7 +583,9 @@ int wlc_mde_backward(wlc_handle* h) {
     if (rc) return rc;
   }
   int rc = sweep(h, -1, Ns - 1, 0, dens);
+  const int rj = join_density(h);  // also on failure: no density launch outlives the call
   if (rc) return rc;
+  if (rj) return rj;
   h->bwd_done = dens;
   if (h->timing) return flush_timing(h);
   return WLC_OK;

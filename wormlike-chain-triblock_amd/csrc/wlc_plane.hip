@@ -22,14 +22,20 @@ constexpr int kChunk = 8;  // plane elements each thread loads per batch
 constexpr int kPairs = 4;  // x pairs each thread loads per batch (kernel A)
 }  // namespace
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
-// Remap the linear block id so that every XCD walks a contiguous range of
-// (harmonic group, z-plane) pairs, z fastest: the z-fastest half spectrum then has
-// the 16-byte pieces of each cache line written/read by workgroups of one XCD at
-// about the same time.
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  When
+// nz is a multiple of 8, XCD k takes the z-planes [k nz/8, (k+1) nz/8) of every
+// harmonic, z fastest: the w planes of its z range are reused from L2 by every
+// harmonic, and the two z-planes that share a half-spectrum cache line run back to
+// back on one XCD.  Otherwise each XCD walks a contiguous range of (harmonic, z).
 __device__ __forceinline__ void plane_of_block(int nz, int& z, int& grp) {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
+  if ((nz & 7) == 0) {
+    const int zr = nz >> 3, t = b >> 3;
+    grp = t / zr;
+    z = (b & 7) * zr + (t - grp * zr);
+    return;
+  }
   const int q = nb >> 3;
   const int logical = (b < (q << 3)) ? ((b & 7) * q + (b >> 3)) : b;
   grp = logical / nz;
@@ -438,31 +444,47 @@ hipError_t launch_fill(double* dst, double v, long long n, hipStream_t st) {
   return hipGetLastError();
 }
 
-// rho_t += w_t * sum_{i < lim_t} q_i qs_i at every grid point, for up to two blocks
-__global__ void k_density_acc(Geom g, const double* __restrict__ q, const double* __restrict__ qs,
-                              double* rho0, int lim0, double w0, double* rho1, int lim1, double w1) {
-  const int nxy = g.nx * g.ny;
+// rho_t += w_t * sum_{i < lim_t} q_i qs_i at every grid point, for up to two blocks.
+// One thread per x pair (16-byte loads); the harmonic loop is unrolled so that many
+// independent loads are in flight, the sum itself runs in harmonic order.
+__global__ void __launch_bounds__(256) k_density_acc(Geom g, const double* __restrict__ q,
+                                                     const double* __restrict__ qs, double* rho0,
+                                                     int lim0, double w0, double* rho1, int lim1,
+                                                     double w1) {
+  const int nxy2 = (g.nx * g.ny) >> 1;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nxy) return;
-  const long long r = (long long)blockIdx.y * g.zs + e;
+  if (e >= nxy2) return;
+  const long long r = (long long)blockIdx.y * g.zs + 2 * e;
   const long long hs = g.hs;
-  double s = 0.0, s0 = 0.0, s1 = 0.0;
+  double2 s = make_double2(0.0, 0.0), s0 = s, s1 = s;
   const int top = lim0 > lim1 ? lim0 : lim1;
+#pragma unroll 8
   for (int i = 0; i < top; ++i) {
-    s += q[i * hs + r] * qs[i * hs + r];
+    const double2 a = *reinterpret_cast<const double2*>(q + i * hs + r);
+    const double2 b = *reinterpret_cast<const double2*>(qs + i * hs + r);
+    s.x = __builtin_fma(a.x, b.x, s.x);
+    s.y = __builtin_fma(a.y, b.y, s.y);
     if (i + 1 == lim0) s0 = s;
     if (i + 1 == lim1) s1 = s;
   }
-  if (rho0) rho0[r] += w0 * s0;
-  if (rho1) rho1[r] += w1 * s1;
+  if (rho0) {
+    double2* p = reinterpret_cast<double2*>(rho0 + r);
+    const double2 o = *p;
+    *p = make_double2(__builtin_fma(w0, s0.x, o.x), __builtin_fma(w0, s0.y, o.y));
+  }
+  if (rho1) {
+    double2* p = reinterpret_cast<double2*>(rho1 + r);
+    const double2 o = *p;
+    *p = make_double2(__builtin_fma(w1, s1.x, o.x), __builtin_fma(w1, s1.y, o.y));
+  }
 }
 
 hipError_t launch_density_acc(const Geom& g, const double* q, const double* qs, double* rho0,
                               int lim0, double w0, double* rho1, int lim1, double w1,
                               hipStream_t st) {
   if (!rho1) lim1 = 0;
-  const int nxy = g.nx * g.ny;
-  dim3 grid((nxy + 255) / 256, g.nz);
+  const int nxy2 = (g.nx * g.ny) / 2;  // nx is even
+  dim3 grid((nxy2 + 255) / 256, g.nz);
   hipLaunchKernelGGL(k_density_acc, grid, dim3(256), 0, st, g, q, qs, rho0, lim0, w0, rho1,
                      lim1, w1);
   return hipGetLastError();

@@ -11,7 +11,7 @@
 //   row role     b = t >> 3 (row pair 0..31), j = t & 7: x = j + 8r
 //   column role  c = t & 31 (kx column 0..31), jy = t >> 5: y = jy + 8r
 //   coalesced    wave w: row pairs 8w..8w+7 as 16-byte pieces (right-hand side, q store)
-// LDS plane (2048 complex, 32 KB, plus a 64-entry scratch column):
+// LDS plane (2048 complex, 32 KB):
 //   x layout  row pair b (rows 2b and 2b+1 as the real and imaginary parts of one
 //             complex row), element x at xs(b, x)
 //   y layout  half-spectrum row y, column c at ys(y, c); column 0 holds
@@ -24,8 +24,8 @@
 namespace wlc {
 namespace p64 {
 
-constexpr int kPlane = 2048;              // complex slots of the plane
-constexpr int kLds = kPlane + 64;         // plus one scratch column
+constexpr int kPlane = 2048;  // complex slots of the plane (32 KB)
+constexpr int kLds = kPlane;
 
 __device__ __forceinline__ int xs(int b, int x) {
   return 64 * b + (x ^ ((x >> 3) & 7) ^ ((b & 1) << 3));
@@ -38,21 +38,16 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// per-thread twiddles exp(-2 pi i j r / 64), r = 1..7, for the x passes (j = t & 7)
-// and the y passes (j = t >> 5); the inverse passes use their conjugates
+// Twiddles exp(-2 pi i j r / 64), r = 1..7, read from the 64-entry table at each pass
+// (j = t & 7 for the x passes, j = t >> 5 for the y passes); the inverse passes use
+// their conjugates.
 struct Tw {
-  double2 x[8];
-  double2 y[8];
+  const double2* __restrict__ t;
+  __device__ __forceinline__ double2 x(int r) const { return t[(threadIdx.x & 7) * r]; }
+  __device__ __forceinline__ double2 y(int r) const { return t[(threadIdx.x >> 5) * r]; }
 };
 
-__device__ __forceinline__ void load_tw(Tw& w, const double2* __restrict__ tw64) {
-  const int j = threadIdx.x & 7, jy = threadIdx.x >> 5;
-#pragma unroll
-  for (int r = 1; r < 8; ++r) {
-    w.x[r] = tw64[j * r];
-    w.y[r] = tw64[jy * r];
-  }
-}
+__device__ __forceinline__ void load_tw(Tw& w, const double2* __restrict__ tw64) { w.t = tw64; }
 
 __device__ __forceinline__ long long sidx(const Geom& g, int i, int ky, int kx, int z) {
   return spec_index(i, ky, kx, z, 64, g.nt, g.nz);
@@ -65,7 +60,7 @@ __device__ __forceinline__ long long sidx(const Geom& g, int i, int ky, int kx, 
 __device__ __forceinline__ void y_inverse(double2* buf, const double2* __restrict__ spec,
                                           const Geom& g, int i, int z, const Tw& w) {
   const int t = threadIdx.x, c = t & 31, jy = t >> 5;
-  double2* hp = buf + kPlane;  // packed column, one entry per ky
+  double2* hp = buf;  // packed column, one entry per ky (before the plane is written)
   double2 v[8];
   if (c != 0) {
 #pragma unroll
@@ -84,6 +79,7 @@ __device__ __forceinline__ void y_inverse(double2* buf, const double2* __restric
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = hp[jy + 8 * r];
   }
+  __syncthreads();  // the scratch column is consumed before the plane overwrites it
   Dft<8, +1>::run(v);
 #pragma unroll
   for (int r = 0; r < 8; ++r) buf[ys(8 * jy + r, c)] = v[r];
@@ -91,7 +87,7 @@ __device__ __forceinline__ void y_inverse(double2* buf, const double2* __restric
 #pragma unroll
   for (int r = 0; r < 8; ++r) v[r] = buf[ys(jy + 8 * r, c)];
 #pragma unroll
-  for (int r = 1; r < 8; ++r) v[r] = twiddle<+1>(v[r], w.y[r]);
+  for (int r = 1; r < 8; ++r) v[r] = twiddle<+1>(v[r], w.y(r));
   Dft<8, +1>::run(v);
 #pragma unroll
   for (int r = 0; r < 8; ++r) buf[ys(jy + 8 * r, c)] = v[r];
@@ -126,7 +122,7 @@ __device__ __forceinline__ void x_inverse(double2* buf, const Tw& w) {
 #pragma unroll
   for (int r = 0; r < 8; ++r) v[r] = buf[xs(b, j + 8 * r)];
 #pragma unroll
-  for (int r = 1; r < 8; ++r) v[r] = twiddle<+1>(v[r], w.x[r]);
+  for (int r = 1; r < 8; ++r) v[r] = twiddle<+1>(v[r], w.x(r));
   Dft<8, +1>::run(v);
   wave_lds_sync();
 #pragma unroll
@@ -150,7 +146,7 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
 #pragma unroll
   for (int r = 0; r < 8; ++r) v[r] = buf[xs(b, j + 8 * r)];
 #pragma unroll
-  for (int r = 1; r < 8; ++r) v[r] = twiddle<-1>(v[r], w.x[r]);
+  for (int r = 1; r < 8; ++r) v[r] = twiddle<-1>(v[r], w.x(r));
   Dft<8, -1>::run(v);
 #pragma unroll
   for (int r = 0; r < 8; ++r) buf[xs(b, j + 8 * r)] = v[r];  // Z_k, k = j + 8r
@@ -179,7 +175,7 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
 #pragma unroll
   for (int r = 0; r < 8; ++r) v[r] = buf[ys(jy + 8 * r, c)];
 #pragma unroll
-  for (int r = 1; r < 8; ++r) v[r] = twiddle<-1>(v[r], w.y[r]);
+  for (int r = 1; r < 8; ++r) v[r] = twiddle<-1>(v[r], w.y(r));
   Dft<8, -1>::run(v);
   if (c != 0) {
 #pragma unroll
