@@ -87,10 +87,17 @@ def cpu_baseline(w, target_s=15.0):
     build_s = time.time() - t0 - first
     n = max(2, min(64, int(target_s / max(first, 1e-3))))
     dt = orc.time_bdf3_steps(w[0], n)
+    # the same steps on one core (a few steps: ~1/3 of the budget)
+    oracle.Oracle.set_threads(1)
+    one = orc.time_bdf3_steps(w[0], 1)
+    n1 = max(1, min(16, int(target_s / 3 / max(one, 1e-3))))
+    dt1 = orc.time_bdf3_steps(w[0], n1)
+    oracle.Oracle.set_threads(threads)
     orc.close()
     return {"value": n / dt, "unit": "contour-steps/s", "cores": threads, "kind": "port",
             "sample": f"{n} BDF3 contour steps of the 64^3 Lmax=6 workload (block A field), "
-                      f"dense G^-1 table build of {build_s:.1f} s excluded"}
+                      f"dense G^-1 table build of {build_s:.1f} s excluded",
+            "single_core": {"value": n1 / dt1, "sample": f"{n1} of the same steps on 1 core"}}
 
 
 def main():

@@ -68,6 +68,8 @@ def lib():
         L.orc_volume_integral.restype = ctypes.c_double
         L.orc_time_bdf3_steps.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
         L.orc_time_bdf3_steps.restype = ctypes.c_double
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
         _LIB = L
     return _LIB
 
@@ -231,6 +233,10 @@ class Oracle:
         if lib().orc_step(self.h, _p(w), direction, s, arr, _p(out)):
             raise RuntimeError("oracle step failed")
         return out
+
+    @staticmethod
+    def set_threads(n):
+        lib().orc_set_threads(int(n))
 
     def time_bdf3_steps(self, w, nsteps):
         w = np.ascontiguousarray(w, dtype=np.float64)

@@ -899,6 +899,10 @@ double orc_free_energy(const orc_ctx* c, const double* w, const double* phi, dou
   return -log(Q) + I;
 }
 
+void orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+}
+
 double orc_time_bdf3_steps(orc_ctx* c, const double* w, int nsteps) {
   int N = c->N;
   int M = c->Mb[0];

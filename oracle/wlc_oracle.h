@@ -77,6 +77,8 @@ double orc_volume_integral(int nx, int ny, int nz, const double* box, const doub
 /* CPU-baseline helper: time nsteps BDF3 steps of block 0 in the forward direction
  * (factor build excluded); returns seconds, or <0 on error. */
 double orc_time_bdf3_steps(orc_ctx* c, const double* w, int nsteps);
+/* OpenMP threads for the following calls (n > 0) */
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

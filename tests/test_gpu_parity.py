@@ -44,6 +44,8 @@ CASES = [
     ((24, 4, 6), (2.0, 2.0, 5.0), (6, 7), (0, 2), (1.0, 0.0), 0),
     ((2, 2, 2), (1.0, 1.0, 1.0), (2, 1, 2), (8, 5, 6), (0.5, 1.0, 3.0), 0),
     ((16, 4, 4), (2.0, 2.0, 2.0), (3, 3, 3), (6, 6, 6), (1.0, 1.0, 1.0), 0),
+    # nz = 64 with Lmax 5..7: the split z-solve kernel (both parts, crossing chains)
+    ((64, 4, 6), (3.0, 2.0, 2.5), (4, 3, 4), (6, 5, 7), (1.0, 0.7, 1.3), 0),
     # 64 x 64 planes (register-resident plane kernels), basis shrinks then grows
     ((4, 64, 64), (2.0, 5.0, 4.0), (3, 4, 3), (3, 2, 3), (1.0, 0.5, 2.0), 0),
     ((2, 64, 64), (1.0, 3.0, 3.0), (5,), (4,), (0.8,), 0),
@@ -168,3 +170,25 @@ def test_fused_inverse_rhs_is_bit_identical(wlc, case, monkeypatch):
         assert np.array_equal(q0[s], q1[s]), f"forward slice {s}"
         assert np.array_equal(qs0[s], qs1[s]), f"backward slice {s}"
     assert np.array_equal(phi0, phi1) and Q0 == Q1 and td0 == td1
+
+
+def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch):
+    """Lmax 5..7 at nz = 64 runs the z solve with the harmonics of a column split over two
+    waves; the one-wave kernel computes the same step (equal up to rounding: the split
+    kernel passes k = 0 through identity-angle rotations instead of a diagonal shortcut)."""
+    shape, box, Nblk, L, lam = (64, 6, 8), (3.0, 2.0, 5.0), (3, 4, 3), (5, 7, 6), (1.2, 0.6, 1.0)
+    rng = np.random.default_rng(31)
+    w = 1.5 * rng.standard_normal((3,) + shape)
+    out = []
+    for split in ("0", "1"):
+        monkeypatch.setenv("WLC_ZSOLVE_SPLIT", split)
+        g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, keep_qstar=True)
+        g.set_w(w)
+        g.MDE_q()
+        g.MDE_qstar()
+        phi, Q, _ = g.density()
+        out.append((g.get_q(+1, sum(Nblk)), g.get_q(-1, 0), phi, Q))
+        g.close()
+    (q0, s0, p0, Q0), (q1, s1, p1, Q1) = out
+    assert _rel(q1, q0) < 1e-13 and _rel(s1, s0) < 1e-13 and _rel(p1, p0) < 1e-13
+    assert abs(Q1 - Q0) / abs(Q0) < 1e-13

@@ -211,12 +211,176 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
   z64_inverse(v, lds, wv, lane, tw, tile, istride);
 }
 
+// ---------------------------------------------------------------------------
+// nz = 64, 5 <= Lmax <= 7: the harmonics of a column split over two waves by degree
+// ---------------------------------------------------------------------------
+// Lo holds l = 0..kLS, Hi holds l = kLS+1..L; each wave keeps only its part in registers,
+// which allows two to three waves per SIMD instead of one (at Lmax = 8 the larger part
+// alone fills a SIMD, so Lmax = 8 keeps the one-wave kernel).  Workgroup = two kx columns of a
+// 4-kx tile (waves: column c = w >> 1, part = w & 1).  The two workgroups of a tile run
+// back to back on one XCD (block order below), so each tile's cache lines are fetched once.
+constexpr int kLS = kL >= 7 ? 5 : (kL == 6 ? 4 : 3);
+using PartLo = SolverPart<kL, 0, kLS>;
+using PartHi = SolverPart<kL, kLS + 1, kL>;
+constexpr int kNV = PartLo::NH > PartHi::NH ? PartLo::NH : PartHi::NH;
+constexpr int kXch = (2 * kLS + 1) * 64;                       // crossing chains x lanes
+constexpr int kRegionS = kRegion > kXch + 2 ? kRegion : kXch + 2;  // per-wave LDS region
+
+// forward z transform of the NH harmonics in v (lane = z -> lane = kz), batches of 8
+template <int NH, int N>
+__device__ __forceinline__ void z64_fwd_regs(double2 (&v)[N], double2* wl, int lane,
+                                             const double2 (&tw)[8]) {
+  const int h = lane >> 3, j = lane & 7;
+#pragma unroll
+  for (int b0 = 0; b0 < NH; b0 += kHB) {
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < NH) wl[q * kPadBatch + pad8(lane)] = v[b0 + q];
+    wave_sync();
+    double2 u[8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
+    Dft<8, -1>::run(u);
+#pragma unroll
+    for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<-1>(u[k2], tw[k2]);
+    wave_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
+    wave_sync();
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
+    Dft<8, -1>::run(u);
+    wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < NH) v[b0 + q] = wl[q * kPadBatch + pad8(lane)];
+    wave_sync();
+  }
+}
+
+// inverse z transform (lane = kz -> lane = z), batches of 8
+template <int NH, int N>
+__device__ __forceinline__ void z64_inv_regs(double2 (&v)[N], double2* wl, int lane,
+                                             const double2 (&tw)[8]) {
+  const int h = lane >> 3, j = lane & 7;
+#pragma unroll
+  for (int b0 = 0; b0 < NH; b0 += kHB) {
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < NH) wl[q * kPadBatch + pad8(lane)] = v[b0 + q];
+    wave_sync();
+    double2 u[8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
+    Dft<8, +1>::run(u);
+#pragma unroll
+    for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<+1>(u[k2], tw[k2]);
+    wave_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
+    wave_sync();
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
+    Dft<8, +1>::run(u);
+    wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < NH) v[b0 + q] = wl[q * kPadBatch + pad8(lane)];
+    wave_sync();
+  }
+}
+
+template <int OCC>
+__global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
+  __shared__ double2 lds[4 * kRegionS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c = wv >> 1, part = wv & 1;
+  const int ny = g.ny, nxh = g.nxh, nt = g.nt;
+  // block -> (tile, half): blocks b and b + 8 (consecutive on XCD b mod 8) share a tile
+  const int nb = gridDim.x, b = blockIdx.x;
+  int tl, half;
+  if ((nb & 15) == 0) {
+    const int t = b >> 3;
+    half = t & 1;
+    tl = (t >> 1) * 8 + (b & 7);
+  } else {
+    half = b & 1;
+    tl = b >> 1;
+  }
+  const int t4 = tl % nt, ky = tl / nt;
+  const int kxl = 2 * half + c;
+  const int kx = t4 * kKxTile + kxl;
+  const bool active = a.solve && kx < nxh;
+  const long long istride = spec_index(1, 0, 0, 0, ny, nt, 64);
+  // column kx of the tile: element (i, z) at i * istride + 4 z (the tile is allocated
+  // even where kx >= nxh)
+  double2* col = a.spec + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, 64) + kxl;
+  double2* wl = lds + wv * kRegionS;
+  double2* xch = lds + (wv & ~1) * kRegionS;  // the Lo wave's region of this column
+  double2 tw[8];
+  {
+    const int j = lane & 7;
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) tw[k2] = g.az.tw[j * k2];
+  }
+  const KFrame f(signed_k(kx, g.nx) * a.kfac[0], signed_k(ky, ny) * a.kfac[1],
+                 signed_k(lane, 64) * a.kfac[2], a.sigma, a.ds);
+  double2 v[kNV];
+  // phase 1: load, forward z transform, rotate into the k frame; Lo: forward elimination
+  if (part == 0) {
+#pragma unroll
+    for (int h = 0; h < PartLo::NH; ++h) v[h] = col[(long long)(PartLo::B0 + h) * istride + 4 * lane];
+    z64_fwd_regs<PartLo::NH>(v, wl, lane, tw);
+    if (active) {
+      PartLo::to_frame(v, f);
+      tri_lower_forward<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < PartHi::NH; ++h) v[h] = col[(long long)(PartHi::B0 + h) * istride + 4 * lane];
+    z64_fwd_regs<PartHi::NH>(v, wl, lane, tw);
+    if (active) PartHi::to_frame(v, f);
+  }
+  __syncthreads();
+  // phase 2: Hi finishes every chain
+  if (part == 1 && active) tri_upper<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+  __syncthreads();
+  // phase 3: Lo back substitution; rotate back, inverse z transform, store
+  if (part == 0) {
+    if (active) {
+      tri_lower_back<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+      PartLo::from_frame(v, f);
+    }
+    z64_inv_regs<PartLo::NH>(v, wl, lane, tw);
+#pragma unroll
+    for (int h = 0; h < PartLo::NH; ++h) col[(long long)(PartLo::B0 + h) * istride + 4 * lane] = v[h];
+  } else {
+    if (active) PartHi::from_frame(v, f);
+    z64_inv_regs<PartHi::NH>(v, wl, lane, tw);
+#pragma unroll
+    for (int h = 0; h < PartHi::NH; ++h) col[(long long)(PartHi::B0 + h) * istride + 4 * lane] = v[h];
+  }
+}
+
 }  // namespace per_l_<L>
 
 using namespace WLC_CAT(per_l_, WLC_L);
 
 hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hipStream_t st) {
   if (a.M != kM) return hipErrorInvalidValue;
+  if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
+    // two workgroups per 4-kx tile
+    const int blocks = 2 * g.nt * g.ny;
+    if (a.occ == 3) hipLaunchKernelGGL(k_zsolve64s<3>, dim3(blocks), dim3(256), 0, st, g, a);
+    else hipLaunchKernelGGL(k_zsolve64s<2>, dim3(blocks), dim3(256), 0, st, g, a);
+    return hipGetLastError();
+  }
   if (g.nz == 64 && !a.generic) {
     static_assert(kKxTile == 4, "one wave per kx of a tile");
     if (a.occ == 2) hipLaunchKernelGGL(k_zsolve64<2>, dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
