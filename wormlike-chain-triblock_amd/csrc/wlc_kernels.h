@@ -73,6 +73,8 @@ hipError_t launch_inv_rhs(const Geom& g, const InvArgs& c, const RhsArgs& a, int
 // small elementwise/reduction kernels
 hipError_t launch_set_delta(const Geom& g, double* slice, int M, hipStream_t st);
 hipError_t launch_fill(double* dst, double v, long long n, hipStream_t st);
+// y += a x over n doubles
+hipError_t launch_axpy(double* y, double a, const double* x, long long n, hipStream_t st);
 // rho_t += wgt_t * sum_{i < lim_t} q_i qs_i for up to two targets (t = 0, 1)
 hipError_t launch_density_acc(const Geom& g, const double* q, const double* qs, double* rho0,
                               int lim0, double w0, double* rho1, int lim1, double w1,

@@ -99,6 +99,20 @@ struct wlc_handle {
   hipEvent_t ev_step = nullptr;
   hipEvent_t ev_dens[4] = {nullptr, nullptr, nullptr, nullptr};
   bool dens_used[4] = {false, false, false, false};
+  // Concurrent sweeps (when the full backward store fits): the backward sweep runs on
+  // stb with its own half spectrum specb while the forward sweep runs on st; slice s of
+  // either sweep records evf[s] / evb[s].  Density term s runs once both slices exist:
+  // on st2 into rho for s <= Ns/2 (ready as the backward sweep reaches s), on st3 into
+  // rho2 for s > Ns/2 (ready as the forward sweep reaches s).  The backward sweep starts
+  // after ev_pre, recorded on st before the forward sweep is queued; every stream is
+  // joined into st before wlc_mde_backward returns.
+  bool conc = false;
+  hipStream_t stb = nullptr, st3 = nullptr;
+  double2* specb = nullptr;
+  double* rho2 = nullptr;
+  std::vector<hipEvent_t> evf, evb;
+  hipEvent_t ev_pre = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
+  bool pre_recorded = false;
   bool have_w = false, fwd_done = false, bwd_done = false;
   int generic_zsolve = 0;
   int zs_occ = 1;
@@ -123,7 +137,7 @@ int block_of(const wlc_handle* h, int s) {
 }
 
 double* qs_slot(wlc_handle* h, int s) {
-  if (h->p.keep_qstar) return h->qs + h->off[s];
+  if (h->p.keep_qstar || h->conc) return h->qs + h->off[s];
   return h->qs + (long long)(s & 3) * h->Mmax * h->hs;
 }
 
@@ -142,6 +156,8 @@ int flush_timing(wlc_handle* h) {
   if (h->recs.empty()) return WLC_OK;
   HIPCHK(hipStreamSynchronize(h->st));
   HIPCHK(hipStreamSynchronize(h->st2));
+  if (h->stb) HIPCHK(hipStreamSynchronize(h->stb));
+  if (h->st3) HIPCHK(hipStreamSynchronize(h->st3));
   for (auto& r : h->recs) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
@@ -195,6 +211,9 @@ hipError_t copy_fields_out(wlc_handle* h, double* dst, const double* src, int nf
   return hipSuccess;
 }
 
+hipStream_t sweep_stream(const wlc_handle* h, int dir) { return (dir < 0 && h->conc) ? h->stb : h->st; }
+double2* sweep_spec(const wlc_handle* h, int dir) { return (dir < 0 && h->conc) ? h->specb : h->spec; }
+
 // Right-hand-side arguments of the step whose target is s.  BDF order by distance from
 // the chain end the sweep starts at (wormlike_MDE_2dcomp_mod.f90:325-343 BDF1, :484-502
 // BDF2, :584-608 BDF3; backward :1046-1064, :1138-1157, :1234-1259); the field is the
@@ -229,7 +248,7 @@ void rhs_args(wlc_handle* h, int dir, int s, wlc::RhsArgs* ra, int* order, doubl
   ra->ds = h->ds;
   ra->w = h->w + (long long)blk * h->hs;
   ra->M = h->Mb[blk];
-  ra->spec = h->spec;
+  ra->spec = sweep_spec(h, dir);
 }
 
 double* slice_of(wlc_handle* h, int dir, int s) { return dir > 0 ? h->q + h->off[s] : qs_slot(h, s); }
@@ -240,8 +259,8 @@ int step_rhs(wlc_handle* h, int dir, int s) {
   int order;
   double cdelta;
   rhs_args(h, dir, s, &ra, &order, &cdelta);
-  TimedLaunch t(h, 0);
-  HIPCHK(wlc::launch_rhs_fft(h->geom, ra, order, h->st));
+  TimedLaunch t(h, 0, sweep_stream(h, dir));
+  HIPCHK(wlc::launch_rhs_fft(h->geom, ra, order, sweep_stream(h, dir)));
   return WLC_OK;
 }
 
@@ -252,7 +271,7 @@ int step_solve(wlc_handle* h, int dir, int s) {
   const int order = dir > 0 ? std::min(s, 3) : std::min(Ns - s, 3);
   const int blk = block_of(h, s);
   wlc::SolveArgs sa{};
-  sa.spec = h->spec;
+  sa.spec = sweep_spec(h, dir);
   sa.M = h->Mb[blk];
   sa.L = h->Lb[blk];
   sa.TX = h->TX[blk];
@@ -266,31 +285,33 @@ int step_solve(wlc_handle* h, int dir, int s) {
   sa.generic = h->generic_zsolve;
   sa.occ = h->zs_occ;
   sa.split = h->zs_split;
-  TimedLaunch t(h, 1);
-  HIPCHK(wlc::launch_zsolve(h->geom, sa, h->st));
+  TimedLaunch t(h, 1, sweep_stream(h, dir));
+  HIPCHK(wlc::launch_zsolve(h->geom, sa, sweep_stream(h, dir)));
   return WLC_OK;
 }
 
 // kernel C for target s (inverse xy, 1/N, store), fused with kernel A of the next
 // target s + dir when `next` is set
 int step_inverse(wlc_handle* h, int dir, int s, bool next) {
-  if (dir < 0 && h->dens_used[s & 3]) HIPCHK(hipStreamWaitEvent(h->st, h->ev_dens[s & 3], 0));
+  const hipStream_t ss = sweep_stream(h, dir);
+  if (dir < 0 && h->dens_used[s & 3]) HIPCHK(hipStreamWaitEvent(ss, h->ev_dens[s & 3], 0));
   wlc::InvArgs ia{};
-  ia.spec = h->spec;
+  ia.spec = sweep_spec(h, dir);
   ia.M = h->Mb[block_of(h, s)];
   ia.scale = 1.0 / (double)h->N;
   ia.out = slice_of(h, dir, s);
   if (!next) {
-    TimedLaunch t(h, 2);
-    HIPCHK(wlc::launch_ifft_store(h->geom, ia, h->st));
-    return WLC_OK;
+    TimedLaunch t(h, 2, ss);
+    HIPCHK(wlc::launch_ifft_store(h->geom, ia, ss));
+  } else {
+    wlc::RhsArgs ra{};
+    int order;
+    double cdelta;
+    rhs_args(h, dir, s + dir, &ra, &order, &cdelta);
+    TimedLaunch t(h, 4, ss);
+    HIPCHK(wlc::launch_inv_rhs(h->geom, ia, ra, order, ss));
   }
-  wlc::RhsArgs ra{};
-  int order;
-  double cdelta;
-  rhs_args(h, dir, s + dir, &ra, &order, &cdelta);
-  TimedLaunch t(h, 4);
-  HIPCHK(wlc::launch_inv_rhs(h->geom, ia, ra, order, h->st));
+  if (h->conc) HIPCHK(hipEventRecord(dir > 0 ? h->evf[s] : h->evb[s], ss));
   return WLC_OK;
 }
 
@@ -313,6 +334,28 @@ int sweep(wlc_handle* h, int dir, int first, int last, bool dens) {
     if (!more) break;
     if (!h->fuse && (rc = step_rhs(h, dir, s + dir))) return rc;
   }
+  return WLC_OK;
+}
+
+// density term s (up to two blocks own slice s) into accumulator `acc` on stream `sd`
+int density_launch(wlc_handle* h, int s, double* acc, hipStream_t sd) {
+  double* rho[2] = {nullptr, nullptr};
+  int lim[2] = {0, 0};
+  double wt[2] = {0.0, 0.0};
+  int n = 0;
+  const int Ms = h->Mb[block_of(h, s)];
+  for (int b = 0; b < h->p.nblk && n < 2; ++b) {
+    const int j = s - h->start[b];
+    if (j < 0 || j > h->p.Nblk[b]) continue;
+    rho[n] = acc + (long long)b * h->hs;
+    lim[n] = std::min(Ms, h->Mb[b]);
+    wt[n] = h->cw[b][j];
+    ++n;
+  }
+  if (n == 0) return WLC_OK;
+  TimedLaunch t(h, 3, sd);
+  HIPCHK(wlc::launch_density_acc(h->geom, h->q + h->off[s], qs_slot(h, s), rho[0], lim[0], wt[0],
+                                 rho[1], lim[1], wt[1], sd));
   return WLC_OK;
 }
 
@@ -358,6 +401,8 @@ void free_handle(wlc_handle* h) {
   (void)hipSetDevice(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   if (h->st2) (void)hipStreamSynchronize(h->st2);
+  if (h->stb) (void)hipStreamSynchronize(h->stb);
+  if (h->st3) (void)hipStreamSynchronize(h->st3);
   for (auto& r : h->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);
   for (auto e : h->uev) if (e) (void)hipEventDestroy(e);
@@ -371,6 +416,14 @@ void free_handle(wlc_handle* h) {
   (void)hipFree(h->scal);
   for (auto t : h->tw) (void)hipFree(t);
   if (h->ev_step) (void)hipEventDestroy(h->ev_step);
+  for (auto e : h->evf) if (e) (void)hipEventDestroy(e);
+  for (auto e : h->evb) if (e) (void)hipEventDestroy(e);
+  if (h->ev_pre) (void)hipEventDestroy(h->ev_pre);
+  for (auto e : h->ev_join) if (e) (void)hipEventDestroy(e);
+  if (h->st3) (void)hipStreamDestroy(h->st3);
+  if (h->stb) (void)hipStreamDestroy(h->stb);
+  (void)hipFree(h->specb);
+  (void)hipFree(h->rho2);
   for (auto e : h->ev_dens) if (e) (void)hipEventDestroy(e);
   if (h->st2) (void)hipStreamDestroy(h->st2);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -407,6 +460,47 @@ int validate(const wlc_params* p) {
 int pick_tx(int nz) {
   // kernel B: one thread per k-point, TX kx columns of nz points per workgroup
   return nz >= 256 ? 1 : 256 / nz;
+}
+
+// backward sweep concurrent with the queued forward sweep (see the handle notes)
+int backward_concurrent(wlc_handle* h) {
+  const int Ns = h->Ns;
+  const bool dens = h->fwd_done;
+  if (!h->pre_recorded) HIPCHK(hipEventRecord(h->ev_pre, h->st));
+  h->pre_recorded = false;
+  HIPCHK(hipStreamWaitEvent(h->stb, h->ev_pre, 0));
+  const long long nacc = (long long)h->p.nblk * h->hs;
+  HIPCHK(wlc::launch_fill(h->rho, 0.0, nacc, h->stb));
+  HIPCHK(wlc::launch_fill(h->rho2, 0.0, nacc, h->stb));
+  HIPCHK(wlc::launch_set_delta(h->geom, qs_slot(h, Ns), h->Mb[block_of(h, Ns)], h->stb));
+  HIPCHK(hipEventRecord(h->evb[Ns], h->stb));
+  int rc = sweep(h, -1, Ns - 1, 0, false);
+  if (rc == WLC_OK && dens) {
+    const int mid = Ns / 2;
+    // s <= Ns/2: ready when the backward sweep reaches s, in decreasing s
+    for (int s = mid; s >= 0 && rc == WLC_OK; --s) {
+      HIPCHK(hipStreamWaitEvent(h->st2, h->evf[s], 0));
+      HIPCHK(hipStreamWaitEvent(h->st2, h->evb[s], 0));
+      rc = density_launch(h, s, h->rho, h->st2);
+    }
+    // s > Ns/2: ready when the forward sweep reaches s, in increasing s
+    for (int s = mid + 1; s <= Ns && rc == WLC_OK; ++s) {
+      HIPCHK(hipStreamWaitEvent(h->st3, h->evf[s], 0));
+      HIPCHK(hipStreamWaitEvent(h->st3, h->evb[s], 0));
+      rc = density_launch(h, s, h->rho2, h->st3);
+    }
+  }
+  // join every stream into st, also on failure
+  const hipStream_t side[3] = {h->stb, h->st2, h->st3};
+  for (int k = 0; k < 3; ++k) {
+    HIPCHK(hipEventRecord(h->ev_join[k], side[k]));
+    HIPCHK(hipStreamWaitEvent(h->st, h->ev_join[k], 0));
+  }
+  if (rc) return rc;
+  if (dens) HIPCHK(wlc::launch_axpy(h->rho, 1.0, h->rho2, nacc, h->st));
+  h->bwd_done = dens;
+  if (h->timing) return flush_timing(h);
+  return WLC_OK;
 }
 
 }  // namespace
@@ -496,10 +590,42 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     free_handle(h);
     return fail(WLC_EHIP, "event creation failed");
   }
-  ALLOC(h->q, sizeof(double) * (size_t)h->off[h->Ns + 1]);
-  if (p->keep_qstar) ALLOC(h->qs, sizeof(double) * (size_t)h->off[h->Ns + 1]);
+  const size_t store_bytes = sizeof(double) * (size_t)h->off[h->Ns + 1];
+  const size_t spec_bytes = sizeof(double2) * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz;
+  ALLOC(h->q, store_bytes);
+  {
+    // concurrent sweeps need the full backward store, a second spectrum and a second
+    // accumulator; take them when they fit with 2 GiB to spare
+    size_t free_b = 0, total_b = 0;
+    const size_t extra = (p->keep_qstar ? 0 : store_bytes) + spec_bytes +
+                         sizeof(double) * (size_t)p->nblk * h->hs;
+    const char* ce = std::getenv("WLC_CONCURRENT");
+    const bool want = !(ce && std::atoi(ce) == 0);
+    h->conc = want && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+              free_b > extra + spec_bytes + (size_t(2) << 30);
+  }
+  if (p->keep_qstar || h->conc) ALLOC(h->qs, store_bytes);
   else ALLOC(h->qs, sizeof(double) * (size_t)4 * h->Mmax * h->hs);
-  ALLOC(h->spec, sizeof(double2) * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz);
+  ALLOC(h->spec, spec_bytes);
+  if (h->conc) {
+    ALLOC(h->specb, spec_bytes);
+    ALLOC(h->rho2, sizeof(double) * (size_t)p->nblk * h->hs);
+    if (hipStreamCreateWithFlags(&h->stb, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->st3, hipStreamNonBlocking) != hipSuccess) {
+      free_handle(h);
+      return fail(WLC_EHIP, "stream creation failed");
+    }
+    h->evf.assign(h->Ns + 1, nullptr);
+    h->evb.assign(h->Ns + 1, nullptr);
+    bool ok = hipEventCreateWithFlags(&h->ev_pre, hipEventDisableTiming) == hipSuccess;
+    for (auto& e : h->ev_join) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    for (auto& e : h->evf) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    for (auto& e : h->evb) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      free_handle(h);
+      return fail(WLC_EHIP, "event creation failed");
+    }
+  }
   ALLOC(h->w, sizeof(double) * (size_t)p->nblk * h->hs);
   ALLOC(h->rho, sizeof(double) * (size_t)p->nblk * h->hs);
   ALLOC(h->phi, sizeof(double) * (size_t)p->nblk * h->hs);
@@ -564,7 +690,12 @@ int wlc_mde_forward(wlc_handle* h) {
   if (!h) return fail(WLC_EINVAL, "NULL handle");
   if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede forward");
   HIPCHK(hipSetDevice(h->device));
+  if (h->conc) {
+    HIPCHK(hipEventRecord(h->ev_pre, h->st));  // the backward sweep may start from here
+    h->pre_recorded = true;
+  }
   HIPCHK(wlc::launch_set_delta(h->geom, h->q, h->Mb[block_of(h, 0)], h->st));
+  if (h->conc) HIPCHK(hipEventRecord(h->evf[0], h->st));
   int rc = sweep(h, +1, 1, h->Ns, false);
   if (rc) return rc;
   h->fwd_done = true;
@@ -578,6 +709,7 @@ int wlc_mde_backward(wlc_handle* h) {
   if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede backward");
   HIPCHK(hipSetDevice(h->device));
   const int Ns = h->Ns;
+  if (h->conc) return backward_concurrent(h);
   HIPCHK(wlc::launch_fill(h->rho, 0.0, (long long)h->p.nblk * h->hs, h->st));
   HIPCHK(wlc::launch_set_delta(h->geom, qs_slot(h, Ns), h->Mb[block_of(h, Ns)], h->st));
   const bool dens = h->fwd_done;

@@ -438,6 +438,18 @@ __global__ void k_fill(double* d, double v, long long n) {
     d[e] = v;
 }
 
+__global__ void k_axpy(double* y, double a, const double* x, long long n) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    y[e] += a * x[e];
+}
+
+hipError_t launch_axpy(double* y, double a, const double* x, long long n, hipStream_t st) {
+  int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(256), 0, st, y, a, x, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(double* dst, double v, long long n, hipStream_t st) {
   int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, dst, v, n);
