@@ -1,5 +1,7 @@
 """The Fortran binding (module wormlike_MDE: MDE_q / MDE_qstar / density / free_energy)
-driven by the simple-mixing SCFT program, against the same loop on the CPU oracle."""
+and the SCFT field update (module scft_iterate: simple, then Anderson mixing) driven by
+the scft_driver program, against the same loop on the CPU oracle with a numpy
+restatement of the update."""
 import os
 import subprocess
 
@@ -8,17 +10,18 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "wormlike-chain-triblock_amd", "fortran", "scft_simple_mixing")
+EXE = os.path.join(ROOT, "wormlike-chain-triblock_amd", "fortran", "scft_driver")
 
 
-def write_input(path, shape, box, Nblk, L, lam, chi, step, niter, w):
+def write_input(path, shape, box, Nblk, L, lam, chi, w, max_iter, method=0, n_simple=20,
+                n_hist=10, lam_s=0.05, lam_a=0.1, tol=1e-6, form=0):
     nz, ny, nx = shape
     nb = len(Nblk)
-    hdr = np.zeros(13, dtype=np.int32)
+    hdr = np.zeros(16, dtype=np.int32)
     hdr[:5] = [nx, ny, nz, nb, sum(Nblk)]
     hdr[5:5 + nb] = Nblk
     hdr[8:8 + nb] = L
-    hdr[11] = niter
+    hdr[11:16] = [max_iter, method, n_simple, n_hist, form]
     lam3 = np.zeros(3)
     lam3[:nb] = lam
     with open(path, "wb") as f:
@@ -26,49 +29,124 @@ def write_input(path, shape, box, Nblk, L, lam, chi, step, niter, w):
         f.write(np.asarray(box, dtype=np.float64).tobytes())
         f.write(lam3.tobytes())
         f.write(np.asarray(chi, dtype=np.float64).reshape(9).tobytes())
-        f.write(np.float64(step).tobytes())
+        f.write(np.asarray([lam_s, lam_a, tol], dtype=np.float64).tobytes())
         # Fortran w_blk(0:N-1, 1:nblk): column-major == [nblk][N] C order
         f.write(np.ascontiguousarray(w, dtype=np.float64).tobytes())
 
 
-def oracle_loop(oracle, shape, box, Nblk, L, lam, chi, step, niter, w):
+class Mixer:
+    """numpy restatement of scft_iterate's update (simple, then Anderson mixing)."""
+
+    def __init__(self, oracle, box, n_simple, n_hist, lam_s, lam_a, form=0):
+        self.o, self.box = oracle, box
+        self.ns, self.nh, self.ls, self.la = n_simple, n_hist, lam_s, lam_a
+        self.form = form  # 0: outputs + lam residuals (reference), 1: inputs + lam residuals
+        self.wout, self.dw = {}, {}
+
+    def vdot(self, a, b):
+        return self.o.volume_integral(a * b, self.box) / float(np.prod(self.box))
+
+    def update(self, it, w, phi, chi, f):
+        nb = len(w)
+        xi = w.mean(0)
+        t = np.array([xi - w[a] + sum(chi[a][b] * (phi[b] - f[b]) for b in range(nb) if b != a)
+                      for a in range(nb)])
+        wo = w + t
+        if it <= self.ns:
+            if it == self.ns:
+                self.wout[0], self.dw[0] = wo, wo - w
+            return w + self.ls * t, np.abs(t).reshape(nb, -1).max(1)
+        k = (it - self.ns) % (self.nh + 1)
+        nr = min(it - self.ns, self.nh)
+        self.wout[k], self.dw[k] = wo, wo - w
+        res = np.array([np.sqrt(abs(self.vdot(self.dw[k][a], self.dw[k][a]) /
+                                    self.vdot(wo[a], wo[a]))) for a in range(nb)])
+        slot = [(k - m) % (self.nh + 1) for m in range(nr + 1)]
+        U = np.zeros((nr, nr))
+        V = np.zeros(nr)
+        for m in range(1, nr + 1):
+            for q in range(1, nr + 1):
+                U[m - 1, q - 1] = sum(self.vdot(self.dw[k][a] - self.dw[slot[m]][a],
+                                                self.dw[k][a] - self.dw[slot[q]][a]) for a in range(nb))
+            V[m - 1] = sum(self.vdot(self.dw[k][a] - self.dw[slot[m]][a], self.dw[k][a])
+                           for a in range(nb))
+        c = np.linalg.solve(U, V)
+        wi = self.wout[k] + sum(c[m - 1] * (self.wout[slot[m]] - self.wout[k]) for m in range(1, nr + 1))
+        di = self.dw[k] + sum(c[m - 1] * (self.dw[slot[m]] - self.dw[k]) for m in range(1, nr + 1))
+        if self.form == 1:
+            wi = wi - di  # combination of the inputs
+        return wi + self.la * di, res
+
+
+def oracle_loop(oracle, shape, box, Nblk, L, lam, chi, w, niter, n_simple, n_hist, lam_s, lam_a,
+                form=0):
     orc = oracle.Oracle(shape, box, Nblk, L, lam)
     f = np.array(Nblk, dtype=float) / sum(Nblk)
-    nb = len(Nblk)
+    mix = Mixer(oracle, box, n_simple, n_hist, lam_s, lam_a, form)
     w = w.copy()
     rows = []
-    for it in range(niter):
+    for it in range(1, niter + 1):
         phi, Q, _ = orc.density(orc.forward(w), orc.backward(w))
         F = orc.free_energy(w, phi, Q, chi)
-        xi = w.mean(0)
-        t = np.zeros_like(w)
-        for a in range(nb):
-            for b in range(nb):
-                if b != a:
-                    t[a] += chi[a][b] * (phi[b] - f[b])
-            t[a] += xi - w[a]
-        rows.append([it + 1, F, np.log(Q), np.abs(t).max()] + [phi[a].mean() for a in range(nb)])
-        w = w + step * t
+        w, res = mix.update(it, w, phi, chi, f)
+        nb = len(Nblk)
+        r3 = np.zeros(3)
+        r3[:nb] = res
+        m3 = np.zeros(3)
+        m3[:nb] = [phi[a].mean() for a in range(nb)]
+        rows.append([it, F, np.log(Q), res.max()] + list(r3) + list(m3))
     return np.array(rows), w
 
 
-@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran binding not built (no flang)")
-def test_fortran_drop_in_scft_loop(wlc, oracle, tmp_path):
-    shape, box = (16, 16, 16), (3.0, 3.0, 3.0)
-    Nblk, L, lam = (4, 5, 3), (3, 2, 3), (1.0, 0.5, 2.0)
-    chi = np.array([[0.0, 20.0, 15.0], [20.0, 0.0, 25.0], [15.0, 25.0, 0.0]])
-    step, niter = 0.05, 3
-    rng = np.random.default_rng(4)
-    w = 0.5 * rng.standard_normal((3,) + shape)
+def run_driver(tmp_path, shape, box, Nblk, L, lam, chi, w, niter, **kw):
     inp, out, wout = tmp_path / "in.bin", tmp_path / "out.txt", tmp_path / "w.bin"
-    write_input(inp, shape, box, Nblk, L, lam, chi, step, niter, w)
-    r = subprocess.run([EXE, str(inp), str(out), str(wout)], capture_output=True, text=True, timeout=120)
+    write_input(inp, shape, box, Nblk, L, lam, chi, w, niter, **kw)
+    r = subprocess.run([EXE, str(inp), str(out), str(wout)], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    got = np.loadtxt(out).reshape(niter, -1)
-    ref, wref = oracle_loop(oracle, shape, box, Nblk, L, lam, chi, step, niter, w)
+    lines = open(out).read().splitlines()
+    rows = np.array([[float(x) for x in ln.split()] for ln in lines if not ln.startswith("#")])
+    trailer = [ln for ln in lines if ln.startswith("#")][-1].split()
+    wfin = np.fromfile(wout, dtype=np.float64).reshape(w.shape)
+    return rows, wfin, trailer
+
+
+CASE = dict(shape=(16, 16, 16), box=(3.0, 3.0, 3.0), Nblk=(4, 5, 3), L=(3, 2, 3),
+            lam=(1.0, 0.5, 2.0))
+CHI = np.array([[0.0, 20.0, 15.0], [20.0, 0.0, 25.0], [15.0, 25.0, 0.0]])
+
+
+def _compare(got, ref, wfin, wref):
     assert np.array_equal(got[:, 0], ref[:, 0])
     assert np.abs(got[:, 1] - ref[:, 1]).max() < 1e-9                  # F, kT per chain
     assert np.abs(got[:, 2] - ref[:, 2]).max() < 1e-10                 # ln Q
-    assert np.abs(got[:, 3:] - ref[:, 3:]).max() < 1e-9 * max(1.0, np.abs(ref[:, 3:]).max())
-    wfin = np.fromfile(wout, dtype=np.float64).reshape(w.shape)
+    assert np.abs(got[:, 3:7] - ref[:, 3:7]).max() < 1e-8 * max(1.0, np.abs(ref[:, 3:7]).max())
+    assert np.abs(got[:, 7:10] - ref[:, 7:10]).max() < 1e-10           # mean phi
     assert np.abs(wfin - wref).max() < 1e-9 * np.abs(wref).max()
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran binding not built (no flang)")
+def test_fortran_simple_mixing_loop(wlc, oracle, tmp_path):
+    rng = np.random.default_rng(4)
+    w = 0.5 * rng.standard_normal((3,) + CASE["shape"])
+    got, wfin, _ = run_driver(tmp_path, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05)
+    ref, wref = oracle_loop(oracle, CASE["shape"], CASE["box"], CASE["Nblk"], CASE["L"],
+                            CASE["lam"], CHI, w, 3, 10 ** 6, 1, 0.05, 0.1)
+    _compare(got, ref, wfin, wref)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran binding not built (no flang)")
+@pytest.mark.parametrize("form", [0, 1])
+def test_fortran_anderson_mixing_loop(wlc, oracle, tmp_path, form):
+    """Two simple-mixing iterations, then Anderson mixing with a 3-deep history (the ring
+    wraps within the 7 iterations); both update forms."""
+    rng = np.random.default_rng(5)
+    w = 0.5 * rng.standard_normal((3,) + CASE["shape"])
+    kw = dict(n_simple=2, n_hist=3, lam_s=0.05, lam_a=0.2)
+    got, wfin, trailer = run_driver(tmp_path, **CASE, chi=CHI, w=w, niter=7, method=1,
+                                    tol=1e-12, form=form, **kw)
+    ref, wref = oracle_loop(oracle, CASE["shape"], CASE["box"], CASE["Nblk"], CASE["L"],
+                            CASE["lam"], CHI, w, 7, kw["n_simple"], kw["n_hist"], kw["lam_s"],
+                            kw["lam_a"], form)
+    _compare(got, ref, wfin, wref)
+    assert trailer[1:5] == ["iterations", "7", "converged", "0"]

@@ -1,5 +1,6 @@
-"""Build the Fortran binding (module wormlike_MDE) and the simple-mixing SCFT driver
-with ROCm flang, linked against libwlc_mde.so (rpath to the package directory)."""
+"""Build the Fortran binding (module wormlike_MDE), the SCFT field update (module
+scft_iterate) and the SCFT driver program with ROCm flang, linked against libwlc_mde.so
+(rpath to the package directory)."""
 import os
 import shutil
 import subprocess
@@ -8,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 BUILD = os.path.join(PKG, "_build", "fortran")
-EXE = os.path.join(HERE, "scft_simple_mixing")
+EXE = os.path.join(HERE, "scft_driver")
 
 
 def find_flang():
@@ -25,15 +26,15 @@ def build():
         print("build_fortran: no Fortran compiler found; skipping the Fortran binding")
         return None
     os.makedirs(BUILD, exist_ok=True)
-    mod_obj = os.path.join(BUILD, "wormlike_mde_gpu.o")
-    drv_obj = os.path.join(BUILD, "scft_simple_mixing.o")
+    objs = []
+    steps = []
+    for src in ("wormlike_mde_gpu.f90", "scft_iterate.f90", "scft_driver.f90"):
+        obj = os.path.join(BUILD, src.replace(".f90", ".o"))
+        steps.append([fc, "-O2", "-module-dir", BUILD, "-I", BUILD, "-c", os.path.join(HERE, src),
+                      "-o", obj])
+        objs.append(obj)
     tmp = EXE + ".tmp"
-    steps = [
-        [fc, "-O2", "-module-dir", BUILD, "-c", os.path.join(HERE, "wormlike_mde_gpu.f90"), "-o", mod_obj],
-        [fc, "-O2", "-module-dir", BUILD, "-I", BUILD, "-c", os.path.join(HERE, "scft_simple_mixing.f90"),
-         "-o", drv_obj],
-        [fc, "-o", tmp, mod_obj, drv_obj, f"-L{PKG}", "-lwlc_mde", "-Wl,-rpath,$ORIGIN/.."],
-    ]
+    steps.append([fc, "-o", tmp] + objs + [f"-L{PKG}", "-lwlc_mde", "-Wl,-rpath,$ORIGIN/.."])
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD)
         if r.returncode != 0:

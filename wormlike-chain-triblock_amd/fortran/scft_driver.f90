@@ -1,0 +1,91 @@
+! Plain SCFT for an AB or ABC wormlike chain, driving the GPU propagator solve through
+! module wormlike_MDE the way the reference driver loop does (SCFT_control.f90:237-262:
+! MDE_q -> MDE_qstar -> density -> free_energy -> field update, until converged or the
+! iteration limit), with simple or simple-then-Anderson mixing (module scft_iterate).
+!
+! usage: scft_driver <input.bin> <output.txt> [final_w.bin]
+! input (stream, native endian):
+!   int32  nx, ny, nz, nblk, Ns, Nblk(3), Lmax(3), max_iter, method (0 simple mixing only,
+!          1 simple then Anderson), n_simple, n_hist, Anderson form (0 reference, 1 mixing
+!          of the inputs; see module scft_iterate)
+!   real64 box(3), lambda(3), chi(3,3) (row a, column b), lam_simple, lam_anderson, tol
+!   real64 w(nx*ny*nz, nblk)
+! output: one line per iteration: iter, F, lnQ, max residual, residual per species (3),
+!   mean phi per block (3), max |w| after the update; then
+!   "# iterations <n> converged <0|1> seconds <wall>"
+!   (wall time of the loop: propagators, densities and field updates)
+program scft_driver
+  use wormlike_MDE
+  use scft_iterate
+  implicit none
+  character(len=1024) :: fin, fout, fw
+  integer :: u, hdr(16), max_iter, method, it, a, b, n, nargs, done
+  integer(8) :: c0, c1, rate
+  real(8) :: box(3), lam(3), chi9(9), chi(3, 3), f(3), tol, mix(2)
+  logical :: converged
+
+  nargs = command_argument_count()
+  if (nargs < 2) then
+     write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin]'
+     error stop 2
+  end if
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  fw = ''
+  if (nargs >= 3) call get_command_argument(3, fw)
+
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) hdr
+  SIDEx = hdr(1); SIDEy = hdr(2); SIDEz = hdr(3)
+  n_blk = hdr(4); Nmax = hdr(5)
+  Ns_blk = hdr(6:8); L_bar_blk = hdr(9:11)
+  max_iter = hdr(12); method = hdr(13)
+  n_simple = hdr(14); n_hist = max(1, hdr(15)); anderson_form = hdr(16)
+  if (method == 0) n_simple = max_iter + 1
+  read(u) box, lam, chi9, mix, tol
+  lam_simple = mix(1); lam_anderson = mix(2)
+  box_x = box(1); box_y = box(2); box_z = box(3)
+  lambda_blk = lam
+  do a = 1, 3
+     do b = 1, 3
+        chi(a, b) = chi9(3 * (a - 1) + b)
+     end do
+  end do
+  call MDE_setup()
+  n = SIDEx * SIDEy * SIDEz
+  read(u) w_blk
+  close(u)
+  call scft_iterate_init()
+  f = 0d0
+  do a = 1, n_blk
+     f(a) = dble(Ns_blk(a)) / dble(Nmax)
+  end do
+
+  open(newunit=u, file=trim(fout), status='replace', action='write')
+  converged = .false.
+  done = 0
+  call system_clock(c0, rate)
+  do it = 1, max_iter
+     call MDE_q(.true.)
+     call MDE_qstar(.true.)
+     call density()
+     call free_energy(chi)
+     call scft_update(it, chi, f, tol, converged)
+     write(u, '(i6, 3es25.16e3, 3es25.16e3, 3es25.16e3, es25.16e3)') it, FE_global, pff_global, &
+          maxval(resid(1:n_blk)), resid, (sum(R_blk(:, a)) / n, a = 1, n_blk), &
+          (0d0, a = n_blk + 1, 3), maxval(abs(w_blk))
+     flush(u)
+     done = it
+     if (converged) exit
+  end do
+  call system_clock(c1)
+  write(u, '(a, i0, a, i0, a, es16.8)') '# iterations ', done, ' converged ', merge(1, 0, converged), &
+       ' seconds ', dble(c1 - c0) / dble(rate)
+  close(u)
+  if (len_trim(fw) > 0) then
+     open(newunit=u, file=trim(fw), access='stream', form='unformatted', status='replace')
+     write(u) w_blk
+     close(u)
+  end if
+  call MDE_release()
+end program scft_driver

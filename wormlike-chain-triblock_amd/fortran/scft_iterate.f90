@@ -1,0 +1,161 @@
+! SCFT field update for an AB or ABC wormlike chain around the GPU propagator module
+! (module wormlike_MDE): simple mixing, then Anderson mixing, with the reference's
+! residuals and convergence test (AB_iterate, SCFT_iterate.f90:140-333, :1146-1150),
+! generalised from two to n_blk species.
+!
+! Forces and outputs per species a (xi = mean_b w_b at each point):
+!   t_a     = sum_{b /= a} chi_ab (phi_b - f_b) + xi - w_a              (simple mixing, :145-147)
+!   Wout_a  = w_a + t_a,  dW_a = t_a                                    (Anderson, :201-204)
+! The reference's two-species Anderson output chiN phi_B + (w_A + w_B - chiN)/2 differs from
+! this by the constant chiN (f_B - 1/2) per species, which shifts the converged fields by a
+! constant and leaves phi and F unchanged; its literal extension to three species
+! (sum_{b /= a} chi_ab (phi_b - 1/2) + xi) has no fixed point with sum_b phi_b = 1.
+! Residual per species: max_r |t_a| while mixing simply (:150-151), and
+! sqrt(<dW_a, dW_a> / <Wout_a, Wout_a>) in the Anderson phase (:206-215), where <a, b> is the
+! periodic-Simpson volume average (dot_product_mpi, :1200-1237).  Converged when every
+! species' residual is below the tolerance (:1146).  The reference's extra exit after 150
+! iterations with residuals below 5e-4 (:1151-1154) is not replicated.
+module scft_iterate
+  use wormlike_MDE
+  implicit none
+
+  integer :: n_simple = 20          ! Num_simple_mixing_AB (global_parameters.f90:132)
+  integer :: n_hist = 10            ! Anderson_nim_AB (read from input in the reference)
+  real(8) :: lam_simple = 0.05d0    ! lanbtWA / lanbtWB
+  real(8) :: lam_anderson = 0.1d0   ! lambda_WAB_anderson (global_parameters.f90:135)
+  ! 0: the reference's update w = sum_m c_m Wout_m + lambda sum_m c_m dW_m (:311-324);
+  ! 1: w = sum_m c_m w_m + lambda sum_m c_m dW_m (mixing of the inputs; the reference's form
+  !    is this one with lambda + 1)
+  integer :: anderson_form = 0
+  real(8) :: resid(3) = 0d0         ! residual per species of the last update
+
+  real(8), allocatable, private :: w_out(:, :, :), d_w(:, :, :)  ! (0:N-1, n_blk, 0:n_hist)
+
+contains
+
+  subroutine scft_iterate_init()
+    integer :: n
+    n = SIDEx * SIDEy * SIDEz
+    if (allocated(w_out)) deallocate(w_out, d_w)
+    allocate(w_out(0:n-1, n_blk, 0:n_hist), d_w(0:n-1, n_blk, 0:n_hist))
+    w_out = 0d0
+    d_w = 0d0
+  end subroutine scft_iterate_init
+
+  ! ring slot of history entry i (Anderson_index, SCFT_iterate.f90:1-24)
+  integer function hist_slot(i)
+    integer, intent(in) :: i
+    hist_slot = modulo(i, n_hist + 1)
+  end function hist_slot
+
+  ! periodic-Simpson volume average of a*b (dot_product_mpi)
+  real(8) function vdot(a, b)
+    real(8), intent(in) :: a(0:), b(0:)
+    vdot = periodic_simpson(a * b) / (box_x * box_y * box_z)
+  end function vdot
+
+  ! LU with partial pivoting and solve of the n x n system a x = b (dgetrf + dgetrs as
+  ! called by lapacksolver, matrix_inverse_mod.f90:129-200); b returns x
+  subroutine lu_solve(a, b, n)
+    integer, intent(in) :: n
+    real(8), intent(inout) :: a(n, n), b(n)
+    integer :: i, j, k, p
+    real(8) :: t, row(n)
+    do k = 1, n
+      p = k
+      do i = k + 1, n
+        if (abs(a(i, k)) > abs(a(p, k))) p = i
+      end do
+      if (p /= k) then
+        row = a(k, :); a(k, :) = a(p, :); a(p, :) = row
+        t = b(k); b(k) = b(p); b(p) = t
+      end if
+      if (a(k, k) == 0d0) cycle
+      do i = k + 1, n
+        t = a(i, k) / a(k, k)
+        a(i, k:n) = a(i, k:n) - t * a(k, k:n)
+        b(i) = b(i) - t * b(k)
+      end do
+    end do
+    do k = n, 1, -1
+      t = b(k)
+      do j = k + 1, n
+        t = t - a(k, j) * b(j)
+      end do
+      if (a(k, k) /= 0d0) then
+        b(k) = t / a(k, k)
+      else
+        b(k) = 0d0
+      end if
+    end do
+  end subroutine lu_solve
+
+  ! One field update after density(): iteration number iter (1-based, the reference's
+  ! n_iter), interaction matrix chi(a, b), block fractions f.  Sets resid and converged.
+  subroutine scft_update(iter, chi, f, tol, converged)
+    integer, intent(in) :: iter
+    real(8), intent(in) :: chi(3, 3), f(3), tol
+    logical, intent(out) :: converged
+    real(8), allocatable :: t(:, :), wo(:, :), xi(:), wi(:, :), di(:, :), u(:, :), v(:)
+    integer :: n, a, b, k, m, q, nr, km, kq
+    n = SIDEx * SIDEy * SIDEz
+    allocate(t(0:n-1, n_blk), wo(0:n-1, n_blk), xi(0:n-1))
+    xi = sum(w_blk(:, 1:n_blk), dim=2) / n_blk
+    do a = 1, n_blk
+      t(:, a) = xi - w_blk(:, a)
+      do b = 1, n_blk
+        if (b /= a) t(:, a) = t(:, a) + chi(a, b) * (R_blk(:, b) - f(b))
+      end do
+      wo(:, a) = w_blk(:, a) + t(:, a)
+    end do
+    resid = 0d0
+    if (iter <= n_simple) then
+      if (iter == n_simple) then  ! first Anderson history entry
+        w_out(:, :, 0) = wo
+        d_w(:, :, 0) = wo - w_blk(:, 1:n_blk)
+      end if
+      do a = 1, n_blk
+        resid(a) = maxval(abs(t(:, a)))
+        w_blk(:, a) = w_blk(:, a) + lam_simple * t(:, a)
+      end do
+    else
+      k = hist_slot(iter - n_simple)
+      nr = min(iter - n_simple, n_hist)
+      w_out(:, :, k) = wo
+      d_w(:, :, k) = wo - w_blk(:, 1:n_blk)
+      do a = 1, n_blk
+        resid(a) = sqrt(abs(vdot(d_w(:, a, k), d_w(:, a, k)) / vdot(w_out(:, a, k), w_out(:, a, k))))
+      end do
+      allocate(u(nr, nr), v(nr), wi(0:n-1, n_blk), di(0:n-1, n_blk))
+      u = 0d0
+      v = 0d0
+      do m = 1, nr
+        km = hist_slot(k - m)
+        do q = m, nr
+          kq = hist_slot(k - q)
+          do a = 1, n_blk
+            u(m, q) = u(m, q) + vdot(d_w(:, a, k) - d_w(:, a, km), d_w(:, a, k) - d_w(:, a, kq))
+          end do
+          u(q, m) = u(m, q)
+        end do
+        do a = 1, n_blk
+          v(m) = v(m) + vdot(d_w(:, a, k) - d_w(:, a, km), d_w(:, a, k))
+        end do
+      end do
+      call lu_solve(u, v, nr)
+      wi = w_out(:, :, k)
+      di = d_w(:, :, k)
+      do m = 1, nr
+        km = hist_slot(k - m)
+        wi = wi + v(m) * (w_out(:, :, km) - w_out(:, :, k))
+        di = di + v(m) * (d_w(:, :, km) - d_w(:, :, k))
+      end do
+      if (anderson_form == 1) wi = wi - di  ! sum_m c_m w_m
+      w_blk(:, 1:n_blk) = wi + lam_anderson * di
+      deallocate(u, v, wi, di)
+    end if
+    converged = all(resid(1:n_blk) < tol)
+    deallocate(t, wo, xi)
+  end subroutine scft_update
+
+end module scft_iterate
