@@ -47,7 +47,12 @@ typedef struct wlc_params {
   int Lmax[WLC_MAX_BLOCKS];       /* harmonic cut-off per block, 0..WLC_MAX_L */
   double lambda[WLC_MAX_BLOCKS];  /* angular coefficient Loa/(2 kappa) per block */
   int keep_qstar;                 /* 1: keep every backward slice (for wlc_mde_get_q) */
-  int reserved[7];                /* must be zero */
+  int ckpt_every;                 /* forward store: 0 = every slice when it fits in device
+                                     memory, else checkpoints every 16 steps; K >= 3 =
+                                     checkpoints every K steps, segments recomputed during
+                                     the backward sweep (no counterpart in the reference,
+                                     which stores every slice, init_array_2dcomp_mod.f90:47-51) */
+  int reserved[6];                /* must be zero */
 } wlc_params;
 
 typedef struct wlc_handle wlc_handle;

@@ -113,6 +113,24 @@ struct wlc_handle {
   std::vector<hipEvent_t> evf, evb;
   hipEvent_t ev_pre = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
   bool pre_recorded = false;
+  // Forward-store checkpointing (ck > 0, when the full store does not fit or on request):
+  // the forward sweep keeps a 4-slot ring `fr` and copies slices jK-2, jK-1, jK of every
+  // segment boundary jK (j = 1 .. nseg-1) to `ckpt` and q(Ns) to `qlast`.  The backward
+  // sweep recomputes segment j = (jK, (j+1)K] (segment 0 = [0, K]) from its checkpoint into
+  // segment buffer j % 2 on stream `str` with its own spectrum `specr`, one segment ahead
+  // of the densities that read it: ev_seg_ready[b] marks a buffer filled, ev_seg_free[b]
+  // (on st2, after the last density of the segment it held) marks it reusable.  The
+  // recomputed slices are bit-identical to the first forward sweep's.
+  int ck = 0, nseg = 0;
+  double* fr = nullptr;
+  double* ckpt = nullptr;
+  double* segbuf[2] = {nullptr, nullptr};
+  double* qlast = nullptr;
+  double2* specr = nullptr;
+  hipStream_t str = nullptr;
+  hipEvent_t ev_seg_ready[2] = {nullptr, nullptr}, ev_seg_free[2] = {nullptr, nullptr};
+  bool seg_busy[2] = {false, false};
+  int dens_seg = -1;  // segment whose densities are being queued
   bool have_w = false, fwd_done = false, bwd_done = false;
   int generic_zsolve = 0;
   int zs_occ = 1;
@@ -129,6 +147,31 @@ struct wlc_handle {
 };
 
 namespace {
+
+// where forward slice s of the sweep being queued lives
+struct FwdLoc {
+  int mode = 0;  // 0: full store, 1: 4-slot ring, 2: segment buffer starting at slice lo
+  double* base = nullptr;
+  int lo = 0;
+};
+
+// one sweep being queued: direction, stream, half spectrum, forward-slice locator
+struct SweepCtx {
+  int dir;
+  hipStream_t st;
+  double2* spec;
+  FwdLoc fl;
+};
+
+long long slot_size(const wlc_handle* h) { return (long long)h->Mmax * h->hs; }
+
+double* fwd_slice(const wlc_handle* h, const FwdLoc& fl, int s) {
+  if (fl.mode == 1) return h->fr + (long long)(s & 3) * slot_size(h);
+  if (fl.mode == 2) return fl.base + (long long)(s - fl.lo) * slot_size(h);
+  return h->q + h->off[s];
+}
+
+double* fwd_last(const wlc_handle* h) { return h->ck ? h->qlast : h->q + h->off[h->Ns]; }
 
 int block_of(const wlc_handle* h, int s) {
   for (int b = 0; b < h->p.nblk; ++b)
@@ -211,21 +254,32 @@ hipError_t copy_fields_out(wlc_handle* h, double* dst, const double* src, int nf
   return hipSuccess;
 }
 
-hipStream_t sweep_stream(const wlc_handle* h, int dir) { return (dir < 0 && h->conc) ? h->stb : h->st; }
-double2* sweep_spec(const wlc_handle* h, int dir) { return (dir < 0 && h->conc) ? h->specb : h->spec; }
+// the main sweep of direction dir (forward: full store or ring; backward: own stream and
+// spectrum when the sweeps run concurrently)
+SweepCtx main_sweep(const wlc_handle* h, int dir) {
+  SweepCtx c{dir, h->st, h->spec, FwdLoc{}};
+  if (dir < 0 && h->conc) {
+    c.st = h->stb;
+    c.spec = h->specb;
+  }
+  if (h->ck) c.fl.mode = 1;
+  return c;
+}
 
 // Right-hand-side arguments of the step whose target is s.  BDF order by distance from
 // the chain end the sweep starts at (wormlike_MDE_2dcomp_mod.f90:325-343 BDF1, :484-502
 // BDF2, :584-608 BDF3; backward :1046-1064, :1138-1157, :1234-1259); the field is the
 // target block's; history j is slice s - dir (j + 1) with its own harmonic count.
-void rhs_args(wlc_handle* h, int dir, int s, wlc::RhsArgs* ra, int* order, double* cdelta) {
+void rhs_args(wlc_handle* h, const SweepCtx& sw, int s, wlc::RhsArgs* ra, int* order,
+              double* cdelta) {
   const int Ns = h->Ns;
+  const int dir = sw.dir;
   *order = dir > 0 ? std::min(s, 3) : std::min(Ns - s, 3);
   const int blk = block_of(h, s);
   for (int j = 0; j < 3; ++j) {
     const int sh = s - dir * (j + 1);
     if (j < *order && sh >= 0 && sh <= Ns) {
-      ra->h[j] = dir > 0 ? h->q + h->off[sh] : qs_slot(h, sh);
+      ra->h[j] = dir > 0 ? fwd_slice(h, sw.fl, sh) : qs_slot(h, sh);
       ra->hM[j] = h->Mb[block_of(h, sh)];
     } else {
       ra->h[j] = nullptr;
@@ -248,30 +302,33 @@ void rhs_args(wlc_handle* h, int dir, int s, wlc::RhsArgs* ra, int* order, doubl
   ra->ds = h->ds;
   ra->w = h->w + (long long)blk * h->hs;
   ra->M = h->Mb[blk];
-  ra->spec = sweep_spec(h, dir);
+  ra->spec = sw.spec;
 }
 
-double* slice_of(wlc_handle* h, int dir, int s) { return dir > 0 ? h->q + h->off[s] : qs_slot(h, s); }
+double* slice_of(wlc_handle* h, const SweepCtx& sw, int s) {
+  return sw.dir > 0 ? fwd_slice(h, sw.fl, s) : qs_slot(h, s);
+}
 
 // kernel A for target s (first step of a sweep, or every step when unfused)
-int step_rhs(wlc_handle* h, int dir, int s) {
+int step_rhs(wlc_handle* h, const SweepCtx& sw, int s) {
   wlc::RhsArgs ra{};
   int order;
   double cdelta;
-  rhs_args(h, dir, s, &ra, &order, &cdelta);
-  TimedLaunch t(h, 0, sweep_stream(h, dir));
-  HIPCHK(wlc::launch_rhs_fft(h->geom, ra, order, sweep_stream(h, dir)));
+  rhs_args(h, sw, s, &ra, &order, &cdelta);
+  TimedLaunch t(h, 0, sw.st);
+  HIPCHK(wlc::launch_rhs_fft(h->geom, ra, order, sw.st));
   return WLC_OK;
 }
 
 // kernel B for target s: z pass, solve of G^{c,sigma}(k) (Gmatrix_2dcomp_mod.f90:475-479),
 // inverse z pass
-int step_solve(wlc_handle* h, int dir, int s) {
+int step_solve(wlc_handle* h, const SweepCtx& sw, int s) {
   const int Ns = h->Ns;
+  const int dir = sw.dir;
   const int order = dir > 0 ? std::min(s, 3) : std::min(Ns - s, 3);
   const int blk = block_of(h, s);
   wlc::SolveArgs sa{};
-  sa.spec = sweep_spec(h, dir);
+  sa.spec = sw.spec;
   sa.M = h->Mb[blk];
   sa.L = h->Lb[blk];
   sa.TX = h->TX[blk];
@@ -285,21 +342,22 @@ int step_solve(wlc_handle* h, int dir, int s) {
   sa.generic = h->generic_zsolve;
   sa.occ = h->zs_occ;
   sa.split = h->zs_split;
-  TimedLaunch t(h, 1, sweep_stream(h, dir));
-  HIPCHK(wlc::launch_zsolve(h->geom, sa, sweep_stream(h, dir)));
+  TimedLaunch t(h, 1, sw.st);
+  HIPCHK(wlc::launch_zsolve(h->geom, sa, sw.st));
   return WLC_OK;
 }
 
 // kernel C for target s (inverse xy, 1/N, store), fused with kernel A of the next
 // target s + dir when `next` is set
-int step_inverse(wlc_handle* h, int dir, int s, bool next) {
-  const hipStream_t ss = sweep_stream(h, dir);
+int step_inverse(wlc_handle* h, const SweepCtx& sw, int s, bool next) {
+  const int dir = sw.dir;
+  const hipStream_t ss = sw.st;
   if (dir < 0 && h->dens_used[s & 3]) HIPCHK(hipStreamWaitEvent(ss, h->ev_dens[s & 3], 0));
   wlc::InvArgs ia{};
-  ia.spec = sweep_spec(h, dir);
+  ia.spec = sw.spec;
   ia.M = h->Mb[block_of(h, s)];
   ia.scale = 1.0 / (double)h->N;
-  ia.out = slice_of(h, dir, s);
+  ia.out = slice_of(h, sw, s);
   if (!next) {
     TimedLaunch t(h, 2, ss);
     HIPCHK(wlc::launch_ifft_store(h->geom, ia, ss));
@@ -307,11 +365,11 @@ int step_inverse(wlc_handle* h, int dir, int s, bool next) {
     wlc::RhsArgs ra{};
     int order;
     double cdelta;
-    rhs_args(h, dir, s + dir, &ra, &order, &cdelta);
+    rhs_args(h, sw, s + dir, &ra, &order, &cdelta);
     TimedLaunch t(h, 4, ss);
     HIPCHK(wlc::launch_inv_rhs(h->geom, ia, ra, order, ss));
   }
-  if (h->conc) HIPCHK(hipEventRecord(dir > 0 ? h->evf[s] : h->evb[s], ss));
+  if (h->conc && sw.fl.mode == 0) HIPCHK(hipEventRecord(dir > 0 ? h->evf[s] : h->evb[s], ss));
   return WLC_OK;
 }
 
@@ -319,26 +377,81 @@ int density_at(wlc_handle* h, int s);
 
 // the steps of one sweep, targets first .. last (inclusive, in direction dir); after
 // each target the optional density accumulation
-int sweep(wlc_handle* h, int dir, int first, int last, bool dens) {
-  int rc = step_rhs(h, dir, first);
+int save_checkpoint(wlc_handle* h, const SweepCtx& sw, int s);
+
+int sweep(wlc_handle* h, const SweepCtx& sw, int first, int last, bool dens) {
+  const int dir = sw.dir;
+  int rc = step_rhs(h, sw, first);
   if (rc) return rc;
   for (int s = first;; s += dir) {
     const bool more = s != last;
-    if ((rc = step_solve(h, dir, s))) return rc;
+    if ((rc = step_solve(h, sw, s))) return rc;
     if (h->fuse) {
-      if ((rc = step_inverse(h, dir, s, more))) return rc;
+      if ((rc = step_inverse(h, sw, s, more))) return rc;
     } else {
-      if ((rc = step_inverse(h, dir, s, false))) return rc;
+      if ((rc = step_inverse(h, sw, s, false))) return rc;
     }
+    if (dir > 0 && sw.fl.mode == 1 && (rc = save_checkpoint(h, sw, s))) return rc;
     if (dens && (rc = density_at(h, s))) return rc;
     if (!more) break;
-    if (!h->fuse && (rc = step_rhs(h, dir, s + dir))) return rc;
+    if (!h->fuse && (rc = step_rhs(h, sw, s + dir))) return rc;
   }
   return WLC_OK;
 }
 
+double* ckpt_slot(const wlc_handle* h, int j, int t) {
+  return h->ckpt + ((long long)(j - 1) * 3 + t) * slot_size(h);
+}
+
+// after forward slice s on the ring: keep the restart history of every segment boundary
+// and the last slice
+int save_checkpoint(wlc_handle* h, const SweepCtx& sw, int s) {
+  const int K = h->ck;
+  const size_t bytes = sizeof(double) * (size_t)h->Mb[block_of(h, s)] * h->hs;
+  // j = floor((s + 2) / K) gives jK - 2 <= s < (j + 1)K - 2: only s <= jK is a restart slice
+  const int j = (s + 2) / K;
+  if (j >= 1 && j <= h->nseg - 1 && s <= j * K)
+    HIPCHK(hipMemcpyAsync(ckpt_slot(h, j, s - (j * K - 2)), fwd_slice(h, sw.fl, s), bytes,
+                          hipMemcpyDeviceToDevice, sw.st));
+  if (s == h->Ns)
+    HIPCHK(hipMemcpyAsync(h->qlast, fwd_slice(h, sw.fl, s), bytes, hipMemcpyDeviceToDevice, sw.st));
+  return WLC_OK;
+}
+
+int seg_of(const wlc_handle* h, int s) { return s == 0 ? 0 : (s - 1) / h->ck; }
+
+FwdLoc seg_loc(const wlc_handle* h, int j) {
+  FwdLoc fl;
+  fl.mode = 2;
+  fl.base = h->segbuf[j & 1];
+  fl.lo = j == 0 ? 0 : j * h->ck - 2;
+  return fl;
+}
+
+// queue the recomputation of forward segment j into buffer j % 2 on str
+int recompute_segment(wlc_handle* h, int j) {
+  const int b = j & 1, K = h->ck;
+  if (h->seg_busy[b]) HIPCHK(hipStreamWaitEvent(h->str, h->ev_seg_free[b], 0));
+  SweepCtx sw{+1, h->str, h->specr, seg_loc(h, j)};
+  if (j == 0) {
+    HIPCHK(wlc::launch_set_delta(h->geom, fwd_slice(h, sw.fl, 0), h->Mb[block_of(h, 0)], h->str));
+  } else {
+    for (int t = 0; t < 3; ++t) {
+      const int s = j * K - 2 + t;
+      const size_t bytes = sizeof(double) * (size_t)h->Mb[block_of(h, s)] * h->hs;
+      HIPCHK(hipMemcpyAsync(fwd_slice(h, sw.fl, s), ckpt_slot(h, j, t), bytes,
+                            hipMemcpyDeviceToDevice, h->str));
+    }
+  }
+  int rc = sweep(h, sw, j == 0 ? 1 : j * K + 1, std::min((j + 1) * K, h->Ns), false);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(h->ev_seg_ready[b], h->str));
+  h->seg_busy[b] = true;
+  return WLC_OK;
+}
+
 // density term s (up to two blocks own slice s) into accumulator `acc` on stream `sd`
-int density_launch(wlc_handle* h, int s, double* acc, hipStream_t sd) {
+int density_launch(wlc_handle* h, int s, const double* qf, double* acc, hipStream_t sd) {
   double* rho[2] = {nullptr, nullptr};
   int lim[2] = {0, 0};
   double wt[2] = {0.0, 0.0};
@@ -354,8 +467,8 @@ int density_launch(wlc_handle* h, int s, double* acc, hipStream_t sd) {
   }
   if (n == 0) return WLC_OK;
   TimedLaunch t(h, 3, sd);
-  HIPCHK(wlc::launch_density_acc(h->geom, h->q + h->off[s], qs_slot(h, s), rho[0], lim[0], wt[0],
-                                 rho[1], lim[1], wt[1], sd));
+  HIPCHK(wlc::launch_density_acc(h->geom, qf, qs_slot(h, s), rho[0], lim[0], wt[0], rho[1],
+                                 lim[1], wt[1], sd));
   return WLC_OK;
 }
 
@@ -374,12 +487,30 @@ int density_at(wlc_handle* h, int s) {
     ++n;
   }
   if (n == 0) return WLC_OK;
+  const double* qf = h->q ? h->q + h->off[s] : nullptr;
+  if (h->ck) {
+    // forward slice s from its recomputed segment; the first density of a segment frees
+    // the buffer of the segment above and queues the recomputation two segments down
+    const int j = seg_of(h, s);
+    if (j != h->dens_seg) {
+      if (h->dens_seg >= 0) {
+        HIPCHK(hipEventRecord(h->ev_seg_free[h->dens_seg & 1], h->st2));
+        if (j - 1 >= 0) {
+          int rc = recompute_segment(h, j - 1);
+          if (rc) return rc;
+        }
+      }
+      HIPCHK(hipStreamWaitEvent(h->st2, h->ev_seg_ready[j & 1], 0));
+      h->dens_seg = j;
+    }
+    qf = fwd_slice(h, seg_loc(h, j), s);
+  }
   HIPCHK(hipEventRecord(h->ev_step, h->st));
   HIPCHK(hipStreamWaitEvent(h->st2, h->ev_step, 0));
   {
     TimedLaunch t(h, 3, h->st2);
-    HIPCHK(wlc::launch_density_acc(h->geom, h->q + h->off[s], qs_slot(h, s), rho[0], lim[0],
-                                   wt[0], rho[1], lim[1], wt[1], h->st2));
+    HIPCHK(wlc::launch_density_acc(h->geom, qf, qs_slot(h, s), rho[0], lim[0], wt[0], rho[1],
+                                   lim[1], wt[1], h->st2));
   }
   HIPCHK(hipEventRecord(h->ev_dens[s & 3], h->st2));
   h->dens_used[s & 3] = true;
@@ -403,6 +534,7 @@ void free_handle(wlc_handle* h) {
   if (h->st2) (void)hipStreamSynchronize(h->st2);
   if (h->stb) (void)hipStreamSynchronize(h->stb);
   if (h->st3) (void)hipStreamSynchronize(h->st3);
+  if (h->str) (void)hipStreamSynchronize(h->str);
   for (auto& r : h->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);
   for (auto e : h->uev) if (e) (void)hipEventDestroy(e);
@@ -424,6 +556,15 @@ void free_handle(wlc_handle* h) {
   if (h->stb) (void)hipStreamDestroy(h->stb);
   (void)hipFree(h->specb);
   (void)hipFree(h->rho2);
+  (void)hipFree(h->fr);
+  (void)hipFree(h->ckpt);
+  (void)hipFree(h->segbuf[0]);
+  (void)hipFree(h->segbuf[1]);
+  (void)hipFree(h->qlast);
+  (void)hipFree(h->specr);
+  for (auto e : h->ev_seg_ready) if (e) (void)hipEventDestroy(e);
+  for (auto e : h->ev_seg_free) if (e) (void)hipEventDestroy(e);
+  if (h->str) (void)hipStreamDestroy(h->str);
   for (auto e : h->ev_dens) if (e) (void)hipEventDestroy(e);
   if (h->st2) (void)hipStreamDestroy(h->st2);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -452,6 +593,8 @@ int validate(const wlc_params* p) {
   }
   if (tot != p->Ns) return fail(WLC_EINVAL, "sum of block lengths must equal Ns");
   if (p->Ns < 3) return fail(WLC_EINVAL, "Ns must be >= 3");
+  if (p->ckpt_every < 0 || (p->ckpt_every > 0 && p->ckpt_every < 3))
+    return fail(WLC_EINVAL, "ckpt_every must be 0 (automatic) or >= 3");
   for (int r : p->reserved)
     if (r) return fail(WLC_EINVAL, "reserved fields must be zero");
   return WLC_OK;
@@ -474,20 +617,20 @@ int backward_concurrent(wlc_handle* h) {
   HIPCHK(wlc::launch_fill(h->rho2, 0.0, nacc, h->stb));
   HIPCHK(wlc::launch_set_delta(h->geom, qs_slot(h, Ns), h->Mb[block_of(h, Ns)], h->stb));
   HIPCHK(hipEventRecord(h->evb[Ns], h->stb));
-  int rc = sweep(h, -1, Ns - 1, 0, false);
+  int rc = sweep(h, main_sweep(h, -1), Ns - 1, 0, false);
   if (rc == WLC_OK && dens) {
     const int mid = Ns / 2;
     // s <= Ns/2: ready when the backward sweep reaches s, in decreasing s
     for (int s = mid; s >= 0 && rc == WLC_OK; --s) {
       HIPCHK(hipStreamWaitEvent(h->st2, h->evf[s], 0));
       HIPCHK(hipStreamWaitEvent(h->st2, h->evb[s], 0));
-      rc = density_launch(h, s, h->rho, h->st2);
+      rc = density_launch(h, s, h->q + h->off[s], h->rho, h->st2);
     }
     // s > Ns/2: ready when the forward sweep reaches s, in increasing s
     for (int s = mid + 1; s <= Ns && rc == WLC_OK; ++s) {
       HIPCHK(hipStreamWaitEvent(h->st3, h->evf[s], 0));
       HIPCHK(hipStreamWaitEvent(h->st3, h->evb[s], 0));
-      rc = density_launch(h, s, h->rho2, h->st3);
+      rc = density_launch(h, s, h->q + h->off[s], h->rho2, h->st3);
     }
   }
   // join every stream into st, also on failure
@@ -592,8 +735,39 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   }
   const size_t store_bytes = sizeof(double) * (size_t)h->off[h->Ns + 1];
   const size_t spec_bytes = sizeof(double2) * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz;
-  ALLOC(h->q, store_bytes);
+  const size_t slot_bytes = sizeof(double) * (size_t)slot_size(h);
+  const size_t field_bytes = sizeof(double) * (size_t)p->nblk * h->hs;
   {
+    // full forward store when it fits (with the backward ring, spectrum, fields and
+    // 2 GiB to spare); otherwise segments of ckpt_every (default 16) contour steps
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const size_t base = 4 * slot_bytes + spec_bytes + 4 * field_bytes + (size_t(2) << 30);
+    if (p->ckpt_every > 0) h->ck = p->ckpt_every;
+    else if (store_bytes + base > free_b) h->ck = 16;
+    if (h->ck) h->nseg = (h->Ns + h->ck - 1) / h->ck;
+  }
+  if (h->ck) {
+    ALLOC(h->fr, 4 * slot_bytes);
+    if (h->nseg > 1) ALLOC(h->ckpt, (size_t)(h->nseg - 1) * 3 * slot_bytes);
+    for (int b = 0; b < 2; ++b) ALLOC(h->segbuf[b], (size_t)(h->ck + 3) * slot_bytes);
+    ALLOC(h->qlast, slot_bytes);
+    ALLOC(h->specr, spec_bytes);
+    if (hipStreamCreateWithFlags(&h->str, hipStreamNonBlocking) != hipSuccess) {
+      free_handle(h);
+      return fail(WLC_EHIP, "stream creation failed");
+    }
+    bool ok = true;
+    for (auto& e : h->ev_seg_ready) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    for (auto& e : h->ev_seg_free) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      free_handle(h);
+      return fail(WLC_EHIP, "event creation failed");
+    }
+  } else {
+    ALLOC(h->q, store_bytes);
+  }
+  if (!h->ck) {
     // concurrent sweeps need the full backward store, a second spectrum and a second
     // accumulator; take them when they fit with 2 GiB to spare
     size_t free_b = 0, total_b = 0;
@@ -694,9 +868,10 @@ int wlc_mde_forward(wlc_handle* h) {
     HIPCHK(hipEventRecord(h->ev_pre, h->st));  // the backward sweep may start from here
     h->pre_recorded = true;
   }
-  HIPCHK(wlc::launch_set_delta(h->geom, h->q, h->Mb[block_of(h, 0)], h->st));
+  const SweepCtx sw = main_sweep(h, +1);
+  HIPCHK(wlc::launch_set_delta(h->geom, fwd_slice(h, sw.fl, 0), h->Mb[block_of(h, 0)], h->st));
   if (h->conc) HIPCHK(hipEventRecord(h->evf[0], h->st));
-  int rc = sweep(h, +1, 1, h->Ns, false);
+  int rc = sweep(h, sw, 1, h->Ns, false);
   if (rc) return rc;
   h->fwd_done = true;
   h->bwd_done = false;
@@ -713,12 +888,24 @@ int wlc_mde_backward(wlc_handle* h) {
   HIPCHK(wlc::launch_fill(h->rho, 0.0, (long long)h->p.nblk * h->hs, h->st));
   HIPCHK(wlc::launch_set_delta(h->geom, qs_slot(h, Ns), h->Mb[block_of(h, Ns)], h->st));
   const bool dens = h->fwd_done;
-  if (dens) {
-    int rc = density_at(h, Ns);
-    if (rc) return rc;
+  int rc = WLC_OK;
+  if (dens && h->ck) {
+    // the recomputation stream starts after everything queued so far on st
+    HIPCHK(hipEventRecord(h->ev_step, h->st));
+    HIPCHK(hipStreamWaitEvent(h->str, h->ev_step, 0));
+    h->dens_seg = -1;
+    h->seg_busy[0] = h->seg_busy[1] = false;
+    const int J = h->nseg - 1;
+    rc = recompute_segment(h, J);
+    if (rc == WLC_OK && J >= 1) rc = recompute_segment(h, J - 1);
   }
-  int rc = sweep(h, -1, Ns - 1, 0, dens);
+  if (rc == WLC_OK && dens) rc = density_at(h, Ns);
+  if (rc == WLC_OK) rc = sweep(h, main_sweep(h, -1), Ns - 1, 0, dens);
   const int rj = join_density(h);  // also on failure: no density launch outlives the call
+  if (h->ck) {
+    HIPCHK(hipEventRecord(h->ev_seg_ready[0], h->str));
+    HIPCHK(hipStreamWaitEvent(h->st, h->ev_seg_ready[0], 0));
+  }
   if (rc) return rc;
   if (rj) return rj;
   h->bwd_done = dens;
@@ -731,7 +918,7 @@ static int density_common(wlc_handle* h, double* Q, double* totden) {
   HIPCHK(hipSetDevice(h->device));
   const double dV = (h->p.box[0] / h->p.nx) * (h->p.box[1] / h->p.ny) * (h->p.box[2] / h->p.nz);
   const double V = h->p.box[0] * h->p.box[1] * h->p.box[2];
-  HIPCHK(wlc::launch_weighted_sum(h->geom, h->q + h->off[h->Ns], 1, dV, h->partial, h->nparts, h->scal, h->st));
+  HIPCHK(wlc::launch_weighted_sum(h->geom, fwd_last(h), 1, dV, h->partial, h->nparts, h->scal, h->st));
   HIPCHK(wlc::launch_weighted_sum(h->geom, h->rho, h->p.nblk, dV, h->partial, h->nparts, h->scal + 1, h->st));
   HIPCHK(wlc::launch_phi(h->geom, h->rho, h->p.nblk, V, h->scal + 1, h->phi, h->st));
   double sc[2];
@@ -773,7 +960,8 @@ int wlc_mde_get_q(wlc_handle* h, int dir, int s, double* q_host) {
   const double* src;
   if (dir > 0) {
     if (!h->fwd_done) return fail(WLC_ESTATE, "no forward sweep");
-    src = h->q + h->off[s];
+    if (h->ck && s != h->Ns) return fail(WLC_ESTATE, "forward slices other than Ns are not stored in checkpoint mode");
+    src = h->ck ? h->qlast : h->q + h->off[s];
   } else {
     if (!h->p.keep_qstar && s != 0) return fail(WLC_ESTATE, "backward slices other than 0 need keep_qstar");
     src = qs_slot(h, s);

@@ -21,7 +21,8 @@ module wlc_mde_c
      integer(c_int) :: ns_blk(3), lmax(3)   ! C: Nblk, Lmax
      real(c_double) :: lambda(3)
      integer(c_int) :: keep_qstar
-     integer(c_int) :: reserved(7)
+     integer(c_int) :: ckpt_every            ! 0: automatic forward store / checkpoints
+     integer(c_int) :: reserved(6)
   end type wlc_params
 
   interface
@@ -142,6 +143,7 @@ contains
        p%lambda(b) = lambda_blk(b)
     end do
     p%keep_qstar = 0
+    p%ckpt_every = 0
     p%reserved = 0
     call wlc_check(wlc_mde_create(p, handle), 'wlc_mde_create')
     if (.not. allocated(w_blk)) allocate(w_blk(0:SIDEx*SIDEy*SIDEz-1, n_blk))

@@ -38,7 +38,8 @@ class WlcParams(ctypes.Structure):
         ("nblk", ctypes.c_int), ("Ns", ctypes.c_int),
         ("Nblk", ctypes.c_int * 3), ("Lmax", ctypes.c_int * 3),
         ("lam", ctypes.c_double * 3),
-        ("keep_qstar", ctypes.c_int), ("reserved", ctypes.c_int * 7),
+        ("keep_qstar", ctypes.c_int), ("ckpt_every", ctypes.c_int),
+        ("reserved", ctypes.c_int * 6),
     ]
 
 
@@ -112,7 +113,7 @@ def block_lengths(Ns, fractions):
 class WormlikeMDE:
     """One chain architecture on one grid; all state lives on the GPU."""
 
-    def __init__(self, shape, box, Nblk, Lmax, lam, keep_qstar=False):
+    def __init__(self, shape, box, Nblk, Lmax, lam, keep_qstar=False, ckpt_every=0):
         load()
         nz, ny, nx = shape
         p = WlcParams()
@@ -127,6 +128,7 @@ class WormlikeMDE:
             p.Lmax[b] = int(Lmax[b])
             p.lam[b] = float(lam[b])
         p.keep_qstar = 1 if keep_qstar else 0
+        p.ckpt_every = int(ckpt_every)
         self.params = p
         self.shape = (nz, ny, nx)
         self.N = nx * ny * nz
