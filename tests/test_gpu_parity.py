@@ -49,6 +49,10 @@ CASES = [
     # 64 x 64 planes (register-resident plane kernels), basis shrinks then grows
     ((4, 64, 64), (2.0, 5.0, 4.0), (3, 4, 3), (3, 2, 3), (1.0, 0.5, 2.0), 0),
     ((2, 64, 64), (1.0, 3.0, 3.0), (5,), (4,), (0.8,), 0),
+    # nz = 128 (generic z kernel, two columns per workgroup) at Lmax 8 and 7
+    ((128, 4, 6), (6.0, 2.0, 2.5), (3, 4), (8, 7), (0.5, 1.0), 0),
+    # 128 x 128 planes (runtime plane kernels, 130 KB of LDS per workgroup)
+    ((4, 128, 128), (2.0, 6.0, 6.0), (3, 3), (2, 1), (0.5, 1.5), 0),
 ]
 
 
