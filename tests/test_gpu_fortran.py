@@ -98,11 +98,11 @@ def oracle_loop(oracle, shape, box, Nblk, L, lam, chi, w, niter, n_simple, n_his
     return np.array(rows), w
 
 
-def run_driver(tmp_path, shape, box, Nblk, L, lam, chi, w, niter, **kw):
+def run_driver(tmp_path, shape, box, Nblk, L, lam, chi, w, niter, flags=(), **kw):
     inp, out, wout = tmp_path / "in.bin", tmp_path / "out.txt", tmp_path / "w.bin"
     write_input(inp, shape, box, Nblk, L, lam, chi, w, niter, **kw)
-    r = subprocess.run([EXE, str(inp), str(out), str(wout)], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([EXE, str(inp), str(out), str(wout)] + list(flags), capture_output=True,
+                       text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = open(out).read().splitlines()
     rows = np.array([[float(x) for x in ln.split()] for ln in lines if not ln.startswith("#")])
@@ -150,3 +150,28 @@ def test_fortran_anderson_mixing_loop(wlc, oracle, tmp_path, form):
                             kw["lam_a"], form)
     _compare(got, ref, wfin, wref)
     assert trailer[1:5] == ["iterations", "7", "converged", "0"]
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran binding not built (no flang)")
+def test_fortran_restart_from_field_file(wlc, tmp_path):
+    """Resume through the reference's field file: 2 simple-mixing iterations ending in
+    W_dump (--dumpW), then 2 more starting from readW (--readW, the input fields are
+    ignored) give the same trajectory and final fields, bit for bit, as 4 iterations in one
+    run."""
+    rng = np.random.default_rng(6)
+    w = 0.5 * rng.standard_normal((3,) + CASE["shape"])
+    kw = dict(method=0, lam_s=0.05)
+    one = tmp_path / "one"
+    two = tmp_path / "two"
+    one.mkdir()
+    two.mkdir()
+    full, wfull, _ = run_driver(one, **CASE, chi=CHI, w=w, niter=4, **kw)
+    first, wfirst, _ = run_driver(two, **CASE, chi=CHI, w=w, niter=2, flags=["--dumpW"], **kw)
+    assert (two / "W0.dat").exists()
+    t = np.loadtxt(two / "W0.dat")
+    assert np.array_equal(np.ascontiguousarray(t[:, 1:].T).reshape(w.shape), wfirst)
+    second, wsecond, _ = run_driver(two, **CASE, chi=CHI, w=np.zeros_like(w), niter=2,
+                                    flags=["--readW"], **kw)
+    assert np.array_equal(first[:, 1:], full[:2, 1:])
+    assert np.array_equal(second[:, 1:], full[2:, 1:])
+    assert np.array_equal(wsecond, wfull)

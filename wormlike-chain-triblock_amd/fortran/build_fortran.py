@@ -1,6 +1,7 @@
 """Build the Fortran binding (module wormlike_MDE), the SCFT field update (module
-scft_iterate) and the SCFT driver program with ROCm flang, linked against libwlc_mde.so
-(rpath to the package directory)."""
+scft_iterate), the field files (module field_io) and the SCFT driver program with ROCm
+flang, linked against libwlc_mde.so (rpath to the package directory); and field_io_check,
+a GPU-free round trip of the field files."""
 import os
 import shutil
 import subprocess
@@ -10,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 BUILD = os.path.join(PKG, "_build", "fortran")
 EXE = os.path.join(HERE, "scft_driver")
+CHECK = os.path.join(HERE, "field_io_check")
 
 
 def find_flang():
@@ -28,19 +30,22 @@ def build():
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     steps = []
-    for src in ("wormlike_mde_gpu.f90", "scft_iterate.f90", "scft_driver.f90"):
+    for src in ("wormlike_mde_gpu.f90", "scft_iterate.f90", "field_io.f90", "scft_driver.f90",
+                "field_io_check.f90"):
         obj = os.path.join(BUILD, src.replace(".f90", ".o"))
         steps.append([fc, "-O2", "-module-dir", BUILD, "-I", BUILD, "-c", os.path.join(HERE, src),
                       "-o", obj])
         objs.append(obj)
-    tmp = EXE + ".tmp"
-    steps.append([fc, "-o", tmp] + objs + [f"-L{PKG}", "-lwlc_mde", "-Wl,-rpath,$ORIGIN/.."])
+    tmp, tmp_check = EXE + ".tmp", CHECK + ".tmp"
+    steps.append([fc, "-o", tmp] + objs[:4] + [f"-L{PKG}", "-lwlc_mde", "-Wl,-rpath,$ORIGIN/.."])
+    steps.append([fc, "-o", tmp_check, objs[2], objs[4]])
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD)
         if r.returncode != 0:
             raise RuntimeError(f"Fortran build failed ({' '.join(cmd)}):\n{r.stderr[-4000:]}")
     os.replace(tmp, EXE)
-    print(f"built {EXE}")
+    os.replace(tmp_check, CHECK)
+    print(f"built {EXE}, {CHECK}")
     return EXE
 
 

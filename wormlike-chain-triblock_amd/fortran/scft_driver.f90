@@ -3,7 +3,11 @@
 ! MDE_q -> MDE_qstar -> density -> free_energy -> field update, until converged or the
 ! iteration limit), with simple or simple-then-Anderson mixing (module scft_iterate).
 !
-! usage: scft_driver <input.bin> <output.txt> [final_w.bin]
+! usage: scft_driver <input.bin> <output.txt> [final_w.bin] [--readW] [--dumpW]
+!   --readW  start from the field file W0.dat in the working directory instead of the
+!            fields of input.bin (the reference's wAB_init_type < 0 -> readW,
+!            SCFT_control.f90:79-83)
+!   --dumpW  write the final fields to W0.dat (W_dump, SCFT_dump.f90:273-293)
 ! input (stream, native endian):
 !   int32  nx, ny, nz, nblk, Ns, Nblk(3), Lmax(3), max_iter, method (0 simple mixing only,
 !          1 simple then Anderson), n_simple, n_hist, Anderson form (0 reference, 1 mixing
@@ -17,22 +21,36 @@
 program scft_driver
   use wormlike_MDE
   use scft_iterate
+  use field_io
   implicit none
-  character(len=1024) :: fin, fout, fw
-  integer :: u, hdr(16), max_iter, method, it, a, b, n, nargs, done
+  character(len=1024) :: fin, fout, fw, arg
+  integer :: u, hdr(16), max_iter, method, it, a, b, n, nargs, done, i, npos, stat
+  logical :: read_w, dump_w
   integer(8) :: c0, c1, rate
   real(8) :: box(3), lam(3), chi9(9), chi(3, 3), f(3), tol, mix(2)
   logical :: converged
 
   nargs = command_argument_count()
-  if (nargs < 2) then
-     write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin]'
+  fin = ''; fout = ''; fw = ''
+  read_w = .false.; dump_w = .false.
+  npos = 0
+  do i = 1, nargs
+     call get_command_argument(i, arg)
+     if (trim(arg) == '--readW') then
+        read_w = .true.
+     else if (trim(arg) == '--dumpW') then
+        dump_w = .true.
+     else
+        npos = npos + 1
+        if (npos == 1) fin = arg
+        if (npos == 2) fout = arg
+        if (npos == 3) fw = arg
+     end if
+  end do
+  if (npos < 2 .or. npos > 3) then
+     write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin] [--readW] [--dumpW]'
      error stop 2
   end if
-  call get_command_argument(1, fin)
-  call get_command_argument(2, fout)
-  fw = ''
-  if (nargs >= 3) call get_command_argument(3, fw)
 
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) hdr
@@ -55,6 +73,13 @@ program scft_driver
   n = SIDEx * SIDEy * SIDEz
   read(u) w_blk
   close(u)
+  if (read_w) then
+     call read_w_file(w_file_name(0), w_blk, n, n_blk, stat)
+     if (stat /= 0) then
+        write(*, '(a, a, a, i0)') 'scft_driver: cannot read ', w_file_name(0), ', status ', stat
+        error stop 1
+     end if
+  end if
   call scft_iterate_init()
   f = 0d0
   do a = 1, n_blk
@@ -82,6 +107,13 @@ program scft_driver
   write(u, '(a, i0, a, i0, a, es16.8)') '# iterations ', done, ' converged ', merge(1, 0, converged), &
        ' seconds ', dble(c1 - c0) / dble(rate)
   close(u)
+  if (dump_w) then
+     call write_w_file(w_file_name(0), w_blk, n, n_blk, stat)
+     if (stat /= 0) then
+        write(*, '(a, a, a, i0)') 'scft_driver: cannot write ', w_file_name(0), ', iostat ', stat
+        error stop 1
+     end if
+  end if
   if (len_trim(fw) > 0) then
      open(newunit=u, file=trim(fw), access='stream', form='unformatted', status='replace')
      write(u) w_blk
