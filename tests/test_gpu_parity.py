@@ -46,6 +46,8 @@ CASES = [
     ((16, 4, 4), (2.0, 2.0, 2.0), (3, 3, 3), (6, 6, 6), (1.0, 1.0, 1.0), 0),
     # nz = 64 with Lmax 5..7: the split z-solve kernel (both parts, crossing chains)
     ((64, 4, 6), (3.0, 2.0, 2.5), (4, 3, 4), (6, 5, 7), (1.0, 0.7, 1.3), 0),
+    # ... with nx/2 + 1 = 5 columns: a half-padded and an all-padding workgroup in the last tile
+    ((64, 6, 8), (3.0, 2.0, 4.0), (3, 4, 3), (7, 6, 5), (0.9, 1.1, 0.6), 0),
     # 64 x 64 planes (register-resident plane kernels), basis shrinks then grows
     ((4, 64, 64), (2.0, 5.0, 4.0), (3, 4, 3), (3, 2, 3), (1.0, 0.5, 2.0), 0),
     ((2, 64, 64), (1.0, 3.0, 3.0), (5,), (4,), (0.8,), 0),

@@ -37,6 +37,7 @@ struct RhsArgs {
   double a[3], b[3];   // P = sum_j a_j q_j - ds w sum_j b_j q_j
   double ds;
   const double* w;     // field of the target block [nz][ny][nx]
+  const double* zero;  // a field of zeros, read in place of a history plane that is zero
   int M;               // harmonics of the target slice
   double2* spec;       // half spectrum, layout of spec_index
 };

@@ -82,6 +82,7 @@ struct wlc_handle {
   double* qs = nullptr;        // backward: full store (keep_qstar) or 4-slot ring
   double2* spec = nullptr;
   double* w = nullptr;
+  double* zero = nullptr;  // one field of zeros: the history plane of a harmonic a slice lacks
   double* rho = nullptr;
   double* phi = nullptr;
   double* partial = nullptr;
@@ -301,6 +302,7 @@ void rhs_args(wlc_handle* h, const SweepCtx& sw, int s, wlc::RhsArgs* ra, int* o
   }
   ra->ds = h->ds;
   ra->w = h->w + (long long)blk * h->hs;
+  ra->zero = h->zero;
   ra->M = h->Mb[blk];
   ra->spec = sw.spec;
 }
@@ -543,6 +545,7 @@ void free_handle(wlc_handle* h) {
   (void)hipFree(h->spec);
   (void)hipFree(h->w);
   (void)hipFree(h->rho);
+  (void)hipFree(h->zero);
   (void)hipFree(h->phi);
   (void)hipFree(h->partial);
   (void)hipFree(h->scal);
@@ -801,6 +804,11 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     }
   }
   ALLOC(h->w, sizeof(double) * (size_t)p->nblk * h->hs);
+  ALLOC(h->zero, sizeof(double) * (size_t)h->hs);
+  if (hipMemset(h->zero, 0, sizeof(double) * (size_t)h->hs) != hipSuccess) {
+    free_handle(h);
+    return fail(WLC_EHIP, "zero field initialisation failed");
+  }
   ALLOC(h->rho, sizeof(double) * (size_t)p->nblk * h->hs);
   ALLOC(h->phi, sizeof(double) * (size_t)p->nblk * h->hs);
   ALLOC(h->partial, sizeof(double) * h->nparts);

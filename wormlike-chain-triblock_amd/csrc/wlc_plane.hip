@@ -213,11 +213,12 @@ __device__ __forceinline__ void rhs_to_lds(double* pk, const Geom& g, const RhsA
   const bool u1 = have1 && (FROM_LDS || i < a.hM[0]);
   const bool u2 = rhs && ORDER >= 2 && i < a.hM[1];
   const bool u3 = rhs && ORDER >= 3 && i < a.hM[2];
+  // a zero history plane is read from the zero field: the loads depend only on `in`
   const double2* w2 = reinterpret_cast<const double2*>(a.w + plane);
   const double2* q1 =
-      reinterpret_cast<const double2*>(u1 && !FROM_LDS ? a.h[0] + i * hs + plane : a.w + plane);
-  const double2* q2 = reinterpret_cast<const double2*>(u2 ? a.h[1] + i * hs + plane : a.w + plane);
-  const double2* q3 = reinterpret_cast<const double2*>(u3 ? a.h[2] + i * hs + plane : a.w + plane);
+      reinterpret_cast<const double2*>(u1 && !FROM_LDS ? a.h[0] + i * hs + plane : a.zero + plane);
+  const double2* q2 = reinterpret_cast<const double2*>(u2 ? a.h[1] + i * hs + plane : a.zero + plane);
+  const double2* q3 = reinterpret_cast<const double2*>(u3 ? a.h[2] + i * hs + plane : a.zero + plane);
   double2* o2 = reinterpret_cast<double2*>(out);
   const double2 zero = make_double2(0.0, 0.0);
   for (int base = 0; base < tot2; base += kPairs * kThreads) {
@@ -226,10 +227,15 @@ __device__ __forceinline__ void rhs_to_lds(double* pk, const Geom& g, const RhsA
     for (int c = 0; c < kPairs; ++c) {
       const int e = base + c * kThreads + threadIdx.x;
       const bool in = e < tot2;
-      v1[c] = (!FROM_LDS && u1 && in) ? q1[e] : zero;
-      v2[c] = (u2 && in) ? q2[e] : zero;
-      v3[c] = (u3 && in) ? q3[e] : zero;
-      ww[c] = (rhs && in) ? w2[e] : zero;
+      v1[c] = v2[c] = v3[c] = ww[c] = zero;
+      if (in) {
+        if (!FROM_LDS) v1[c] = q1[e];
+        if (rhs) {
+          if (ORDER >= 2) v2[c] = q2[e];
+          if (ORDER >= 3) v3[c] = q3[e];
+          ww[c] = w2[e];
+        }
+      }
     }
 #pragma unroll
     for (int c = 0; c < kPairs; ++c) {

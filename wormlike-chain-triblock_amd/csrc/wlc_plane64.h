@@ -49,9 +49,20 @@ struct Tw {
 
 __device__ __forceinline__ void load_tw(Tw& w, const double2* __restrict__ tw64) { w.t = tw64; }
 
-__device__ __forceinline__ long long sidx(const Geom& g, int i, int ky, int kx, int z) {
-  return spec_index(i, ky, kx, z, 64, g.nt, g.nz);
-}
+// Spectrum plane (i, z) of the half spectrum: a workgroup-uniform base pointer and
+// 32-bit element offsets (ky stride 4 nt nz, kx-tile stride 4 nz), so the per-thread
+// address arithmetic stays in 32 bits.
+struct SpecPlane {
+  double2* p;
+  unsigned rs, ts;
+  __device__ __forceinline__ SpecPlane(double2* spec, const Geom& g, int i, int z)
+      : p(spec + spec_index(i, 0, 0, z, 64, g.nt, g.nz)),
+        rs(unsigned(g.nt) * unsigned(g.nz) * kKxTile),
+        ts(unsigned(g.nz) * kKxTile) {}
+  __device__ __forceinline__ unsigned at(int ky, int kx) const {
+    return unsigned(ky) * rs + unsigned(kx >> 2) * ts + unsigned(kx & 3);
+  }
+};
 
 // Inverse y pass of half-spectrum plane (i, z) into the y layout (unscaled).  Columns
 // kx = 0 and 32 enter as H0 + i H32 with H(ky) = (X(ky) + conj X(-ky)) / 2: the
@@ -61,15 +72,17 @@ __device__ __forceinline__ void y_inverse(double2* buf, const double2* __restric
                                           const Geom& g, int i, int z, const Tw& w) {
   const int t = threadIdx.x, c = t & 31, jy = t >> 5;
   double2* hp = buf;  // packed column, one entry per ky (before the plane is written)
+  const SpecPlane sp(const_cast<double2*>(spec), g, i, z);
   double2 v[8];
   if (c != 0) {
+    const double2* col = sp.p + sp.at(jy, c);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = spec[sidx(g, i, jy + 8 * r, c, z)];
+    for (int r = 0; r < 8; ++r) v[r] = col[unsigned(8 * r) * sp.rs];
   }
   if (t < 64) {
     const int ky = t, kn = (64 - t) & 63;
-    const double2 a = spec[sidx(g, i, ky, 0, z)], an = spec[sidx(g, i, kn, 0, z)];
-    const double2 b = spec[sidx(g, i, ky, 32, z)], bn = spec[sidx(g, i, kn, 32, z)];
+    const double2 a = sp.p[sp.at(ky, 0)], an = sp.p[sp.at(kn, 0)];
+    const double2 b = sp.p[sp.at(ky, 32)], bn = sp.p[sp.at(kn, 32)];
     const double2 h0 = make_double2(0.5 * (a.x + an.x), 0.5 * (a.y - an.y));
     const double2 h1 = make_double2(0.5 * (b.x + bn.x), 0.5 * (b.y - bn.y));
     hp[ky] = make_double2(h0.x - h1.y, h0.y + h1.x);
@@ -177,9 +190,11 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
 #pragma unroll
   for (int r = 1; r < 8; ++r) v[r] = twiddle<-1>(v[r], w.y(r));
   Dft<8, -1>::run(v);
+  const SpecPlane sp(spec, g, i, z);
   if (c != 0) {
+    double2* col = sp.p + sp.at(jy, c);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) spec[sidx(g, i, jy + 8 * r, c, z)] = v[r];
+    for (int r = 0; r < 8; ++r) col[unsigned(8 * r) * sp.rs] = v[r];
   }
   __syncthreads();  // the plane buffer is free: exchange the packed column
   if (c == 0) {
@@ -192,8 +207,8 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
     // X32 = (P - conj P(-ky)) / 2i
     const int ky = threadIdx.x;
     const double2 p = buf[ky], pn = buf[(64 - ky) & 63];
-    spec[sidx(g, i, ky, 0, z)] = make_double2(0.5 * (p.x + pn.x), 0.5 * (p.y - pn.y));
-    spec[sidx(g, i, ky, 32, z)] = make_double2(0.5 * (p.y + pn.y), -0.5 * (p.x - pn.x));
+    sp.p[sp.at(ky, 0)] = make_double2(0.5 * (p.x + pn.x), 0.5 * (p.y - pn.y));
+    sp.p[sp.at(ky, 32)] = make_double2(0.5 * (p.y + pn.y), -0.5 * (p.x - pn.x));
   }
 }
 
@@ -211,47 +226,59 @@ __device__ __forceinline__ void rhs_rows(double2* buf, const RhsArgs& a, const G
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long hs = g.hs;
   const bool u1 = have1 && (FROM_LDS || i < a.hM[0]);
-  const bool u2 = rhs && ORDER >= 2 && i < a.hM[1];
-  const bool u3 = rhs && ORDER >= 3 && i < a.hM[2];
+  const bool u2 = ORDER >= 2 && i < a.hM[1];
+  const bool u3 = ORDER >= 3 && i < a.hM[2];
+  // a history plane that is zero (beyond the BDF order, or a harmonic the slice lacks)
+  // is read from the zero field, so every load is unconditional and the loads of all
+  // four steps can be in flight together
   const double* w = a.w + plane;
-  const double* h0 = (!FROM_LDS && u1) ? a.h[0] + i * hs + plane : w;
-  const double* h1 = u2 ? a.h[1] + i * hs + plane : w;
-  const double* h2 = u3 ? a.h[2] + i * hs + plane : w;
+  const double* h0 = (!FROM_LDS && u1) ? a.h[0] + i * hs + plane : a.zero + plane;
+  const double* h1 = u2 ? a.h[1] + i * hs + plane : a.zero + plane;
+  const double* h2 = u3 ? a.h[2] + i * hs + plane : a.zero + plane;
   const double2 zero = make_double2(0.0, 0.0);
   const int x = 2 * (lane & 31);
+  const double a0 = a.a[0], a1 = a.a[1], a2 = a.a[2], b0 = a.b[0], b1 = a.b[1], b2 = a.b[2];
+  const double ds = a.ds;
+  double2 q0[4], q1[4];  // (x, x+1) of rows 2b, 2b+1 in step m
+  if (FROM_LDS) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int b = 8 * wv + 2 * m + (lane >> 5);
+      q0[m] = q1[m] = zero;
+      if (u1) {
+        const double2 z0 = buf[xs(b, x)], z1 = buf[xs(b, x + 1)];
+        q0[m] = make_double2(z0.x * scale, z1.x * scale);
+        q1[m] = make_double2(z0.y * scale, z1.y * scale);
+      }
+      if (store) {
+        const int o0 = (2 * b) * 64 + x;
+        *reinterpret_cast<double2*>(out + o0) = q0[m];
+        *reinterpret_cast<double2*>(out + o0 + 64) = q1[m];
+      }
+    }
+  }
+  if (!rhs) {
+    wave_lds_sync();
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const int b = 8 * wv + 2 * m + (lane >> 5);
     const int o0 = (2 * b) * 64 + x, o1 = o0 + 64;  // rows 2b, 2b+1 at x, x+1
-    double2 q0 = zero, q1 = zero;                    // (x, x+1) of rows 2b, 2b+1
-    if (u1) {
-      if (FROM_LDS) {
-        const double2 z0 = buf[xs(b, x)], z1 = buf[xs(b, x + 1)];
-        q0 = make_double2(z0.x * scale, z1.x * scale);
-        q1 = make_double2(z0.y * scale, z1.y * scale);
-      } else {
-        q0 = *reinterpret_cast<const double2*>(h0 + o0);
-        q1 = *reinterpret_cast<const double2*>(h0 + o1);
-      }
+    if (!FROM_LDS) {
+      q0[m] = *reinterpret_cast<const double2*>(h0 + o0);
+      q1[m] = *reinterpret_cast<const double2*>(h0 + o1);
     }
-    if (store) {
-      *reinterpret_cast<double2*>(out + o0) = q0;
-      *reinterpret_cast<double2*>(out + o1) = q1;
-    }
-    if (rhs) {
-      const double2 p0 = u2 ? *reinterpret_cast<const double2*>(h1 + o0) : zero;
-      const double2 p1 = u2 ? *reinterpret_cast<const double2*>(h1 + o1) : zero;
-      const double2 s0 = u3 ? *reinterpret_cast<const double2*>(h2 + o0) : zero;
-      const double2 s1 = u3 ? *reinterpret_cast<const double2*>(h2 + o1) : zero;
-      const double2 w0 = *reinterpret_cast<const double2*>(w + o0);
-      const double2 w1 = *reinterpret_cast<const double2*>(w + o1);
-      const double a0 = a.a[0], a1 = a.a[1], a2 = a.a[2], b0 = a.b[0], b1 = a.b[1], b2 = a.b[2];
-      const double ds = a.ds;
-      buf[xs(b, x)] = make_double2(bdf_rhs(q0.x, p0.x, s0.x, w0.x, a0, a1, a2, b0, b1, b2, ds),
-                                   bdf_rhs(q1.x, p1.x, s1.x, w1.x, a0, a1, a2, b0, b1, b2, ds));
-      buf[xs(b, x + 1)] = make_double2(bdf_rhs(q0.y, p0.y, s0.y, w0.y, a0, a1, a2, b0, b1, b2, ds),
-                                       bdf_rhs(q1.y, p1.y, s1.y, w1.y, a0, a1, a2, b0, b1, b2, ds));
-    }
+    const double2 p0 = ORDER >= 2 ? *reinterpret_cast<const double2*>(h1 + o0) : zero;
+    const double2 p1 = ORDER >= 2 ? *reinterpret_cast<const double2*>(h1 + o1) : zero;
+    const double2 s0 = ORDER >= 3 ? *reinterpret_cast<const double2*>(h2 + o0) : zero;
+    const double2 s1 = ORDER >= 3 ? *reinterpret_cast<const double2*>(h2 + o1) : zero;
+    const double2 w0 = *reinterpret_cast<const double2*>(w + o0);
+    const double2 w1 = *reinterpret_cast<const double2*>(w + o1);
+    buf[xs(b, x)] = make_double2(bdf_rhs(q0[m].x, p0.x, s0.x, w0.x, a0, a1, a2, b0, b1, b2, ds),
+                                 bdf_rhs(q1[m].x, p1.x, s1.x, w1.x, a0, a1, a2, b0, b1, b2, ds));
+    buf[xs(b, x + 1)] = make_double2(bdf_rhs(q0[m].y, p0.y, s0.y, w0.y, a0, a1, a2, b0, b1, b2, ds),
+                                     bdf_rhs(q1[m].y, p1.y, s1.y, w1.y, a0, a1, a2, b0, b1, b2, ds));
   }
   wave_lds_sync();
 }

@@ -314,9 +314,13 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
     tl = b >> 1;
   }
   const int t4 = tl % nt, ky = tl / nt;
+  // both columns of this workgroup lie in the padding of the last tile (kx >= nx/2 + 1):
+  // nothing to do (the condition is uniform over the workgroup, before any barrier)
+  if (t4 * kKxTile + 2 * half >= nxh) return;
   const int kxl = 2 * half + c;
   const int kx = t4 * kKxTile + kxl;
-  const bool active = a.solve && kx < nxh;
+  const bool real_col = kx < nxh;  // wave-uniform; padding columns are neither read nor written
+  const bool active = a.solve && real_col;
   const long long istride = spec_index(1, 0, 0, 0, ny, nt, 64);
   // column kx of the tile: element (i, z) at i * istride + 4 z (the tile is allocated
   // even where kx >= nxh)
@@ -333,7 +337,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
                  signed_k(lane, 64) * a.kfac[2], a.sigma, a.ds);
   double2 v[kNV];
   // phase 1: load, forward z transform, rotate into the k frame; Lo: forward elimination
-  if (part == 0) {
+  if (part == 0 && real_col) {
 #pragma unroll
     for (int h = 0; h < PartLo::NH; ++h) v[h] = col[(long long)(PartLo::B0 + h) * istride + 4 * lane];
     z64_fwd_regs<PartLo::NH>(v, wl, lane, tw);
@@ -341,7 +345,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
       PartLo::to_frame(v, f);
       tri_lower_forward<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
     }
-  } else {
+  } else if (real_col) {
 #pragma unroll
     for (int h = 0; h < PartHi::NH; ++h) v[h] = col[(long long)(PartHi::B0 + h) * istride + 4 * lane];
     z64_fwd_regs<PartHi::NH>(v, wl, lane, tw);
@@ -352,7 +356,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
   if (part == 1 && active) tri_upper<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
   __syncthreads();
   // phase 3: Lo back substitution; rotate back, inverse z transform, store
-  if (part == 0) {
+  if (part == 0 && real_col) {
     if (active) {
       tri_lower_back<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
       PartLo::from_frame(v, f);
@@ -360,7 +364,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
     z64_inv_regs<PartLo::NH>(v, wl, lane, tw);
 #pragma unroll
     for (int h = 0; h < PartLo::NH; ++h) col[(long long)(PartLo::B0 + h) * istride + 4 * lane] = v[h];
-  } else {
+  } else if (real_col) {
     if (active) PartHi::from_frame(v, f);
     z64_inv_regs<PartHi::NH>(v, wl, lane, tw);
 #pragma unroll
