@@ -51,10 +51,15 @@ CASES = [
     # 64 x 64 planes (register-resident plane kernels), basis shrinks then grows
     ((4, 64, 64), (2.0, 5.0, 4.0), (3, 4, 3), (3, 2, 3), (1.0, 0.5, 2.0), 0),
     ((2, 64, 64), (1.0, 3.0, 3.0), (5,), (4,), (0.8,), 0),
-    # nz = 128 (generic z kernel, two columns per workgroup) at Lmax 8 and 7
+    # nz = 128 (two waves per column, radix-2 step across them) at Lmax 8 and 7, and the
+    # generic z kernel on the same case
     ((128, 4, 6), (6.0, 2.0, 2.5), (3, 4), (8, 7), (0.5, 1.0), 0),
-    # 128 x 128 planes (runtime plane kernels, 130 KB of LDS per workgroup)
+    ((128, 4, 6), (6.0, 2.0, 2.5), (3, 4), (8, 7), (0.5, 1.0), 1),
+    # ... nx/2 + 1 = 5 columns (a half-padded and an all-padding workgroup), Lmax 5 and 3
+    ((128, 4, 8), (5.0, 2.0, 3.0), (4, 3), (5, 3), (1.0, 0.4), 0),
+    # 128 x 128 planes (128 KB of LDS per workgroup, radix 16 x 8 passes)
     ((4, 128, 128), (2.0, 6.0, 6.0), (3, 3), (2, 1), (0.5, 1.5), 0),
+    ((2, 128, 128), (1.0, 4.0, 5.0), (2, 3, 2), (3, 2, 3), (1.0, 0.5, 2.0), 0),
 ]
 
 
@@ -149,6 +154,8 @@ FUSE_CASES = [
     ((6, 64, 64), (5.0, 5.0, 2.0), (4, 3, 5), (6, 2, 5), (1.0, 1.0, 1.0)),
     # runtime plane sizes
     ((12, 6, 18), (2.0, 2.5, 3.0), (3, 4, 4), (2, 3, 1), (0.8, 1.2, 0.6)),
+    # 128x128 planes (dynamic 128 KB LDS, 512 threads), basis grows then shrinks
+    ((2, 128, 128), (2.0, 6.0, 5.0), (3, 3, 3), (1, 3, 2), (0.7, 1.0, 1.4)),
 ]
 
 

@@ -349,6 +349,7 @@ static size_t plane_lds_bytes(const Geom& g) {
 
 }  // namespace wlc
 #include "wlc_plane64.h"
+#include "wlc_plane128.h"
 namespace wlc {
 
 template <int ORDER>
@@ -356,9 +357,15 @@ static void* p64_kernel(int kind) {
   return kind == 0 ? (void*)p64::k_rhs64<ORDER> : (void*)p64::k_inv_rhs64<ORDER>;
 }
 
-// kind 0: A (order), 1: C, 2: fused C + A (order of the next step)
-static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds) {
+template <int ORDER>
+static void* p128_kernel(int kind) {
+  return kind == 0 ? (void*)p128::k_rhs128<ORDER> : (void*)p128::k_inv_rhs128<ORDER>;
+}
+
+// kind 0: A (order), 1: C, 2: fused C + A (order of the next step); *threads = workgroup size
+static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds, int* threads) {
   *lds = plane_lds_bytes(g);
+  *threads = kThreads;
   if (g.nx == 64 && g.ny == 64) {  // static 32 KB plane
     *lds = 0;
     if (kind == 1) return (void*)p64::k_inv64;
@@ -366,6 +373,17 @@ static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds) 
       case 1: return p64_kernel<1>(kind);
       case 2: return p64_kernel<2>(kind);
       case 3: return p64_kernel<3>(kind);
+    }
+    return nullptr;
+  }
+  if (g.nx == 128 && g.ny == 128) {  // 128 KB plane in dynamic LDS, 512 threads
+    *lds = sizeof(double2) * p128::kPlane;
+    *threads = p128::kThreads;
+    if (kind == 1) return (void*)p128::k_inv128;
+    switch (order) {
+      case 1: return p128_kernel<1>(kind);
+      case 2: return p128_kernel<2>(kind);
+      case 3: return p128_kernel<3>(kind);
     }
     return nullptr;
   }
@@ -386,13 +404,14 @@ static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds) 
 static hipError_t launch_plane(const Geom& g, int kind, int order, int planes, void** args,
                                hipStream_t st) {
   size_t lds = 0;
-  void* k = pick_plane_kernel(g, kind, order, &lds);
+  int threads = kThreads;
+  void* k = pick_plane_kernel(g, kind, order, &lds, &threads);
   if (!k) return hipErrorInvalidValue;
   if (lds) {
     hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  return hipLaunchKernel(k, dim3(g.nz * planes), dim3(kThreads), args, lds, st);
+  return hipLaunchKernel(k, dim3(g.nz * planes), dim3(threads), args, lds, st);
 }
 
 hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_t st) {

@@ -372,6 +372,114 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// nz = 128: two waves per (kx, ky) column
+// ---------------------------------------------------------------------------
+// Wave p of a column loads z = m + 64 p (lane m).  Forward: one radix-2 decimation-in-
+// frequency step across the two waves (through LDS), a = x(m) + x(m + 64),
+// b = (x(m) - x(m + 64)) w^m with w = exp(-2 pi i / 128), then the wave-local 64-point
+// transform, after which lane m of wave p holds kz = 2 m + p for all harmonics (the
+// solve needs nothing else).  Inverse: wave-local 64-point inverse, then
+// x(m) = a'(m) + w^-m b'(m), x(m + 64) = a'(m) - w^-m b'(m).  Workgroup = two columns.
+constexpr int kXB = 16;  // harmonics per cross-wave exchange
+constexpr int kRegion128 = kRegion > kXB * 64 ? kRegion : kXB * 64;
+
+// exchange with the partner wave of the column: own values in, partner's values out,
+// kXB harmonics at a time; every wave of the workgroup takes part in the barriers
+template <int DIR>
+__device__ __forceinline__ void z128_exchange(double2 (&v)[kM], double2* own, const double2* other,
+                                              int lane, int p, double2 t, bool real_col) {
+#pragma unroll
+  for (int b0 = 0; b0 < kM; b0 += kXB) {
+    __syncthreads();  // the partner has read the previous batch
+    if (real_col) {
+#pragma unroll
+      for (int q = 0; q < kXB; ++q)
+        if (b0 + q < kM) own[q * 64 + lane] = v[b0 + q];
+    }
+    __syncthreads();
+    if (real_col) {
+#pragma unroll
+      for (int q = 0; q < kXB; ++q) {
+        if (b0 + q < kM) {
+          const double2 o = other[q * 64 + lane], x = v[b0 + q];
+          if (DIR < 0) {
+            // forward: wave 0 keeps x(m) + x(m + 64), wave 1 (x(m) - x(m + 64)) w^m
+            v[b0 + q] = p == 0 ? make_double2(x.x + o.x, x.y + o.y)
+                               : twiddle<-1>(make_double2(o.x - x.x, o.y - x.y), t);
+          } else {
+            // inverse: wave 1 arrives holding w^-m b'(m)
+            v[b0 + q] = p == 0 ? make_double2(x.x + o.x, x.y + o.y)
+                               : make_double2(o.x - x.x, o.y - x.y);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int OCC>
+__global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
+  __shared__ double2 lds[4 * kRegion128];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c = wv >> 1, p = wv & 1;
+  const int ny = g.ny, nxh = g.nxh, nt = g.nt;
+  // block -> (tile, half): blocks b and b + 8 (consecutive on XCD b mod 8) share a tile
+  const int nb = gridDim.x, b = blockIdx.x;
+  int tl, half;
+  if ((nb & 15) == 0) {
+    const int t = b >> 3;
+    half = t & 1;
+    tl = (t >> 1) * 8 + (b & 7);
+  } else {
+    half = b & 1;
+    tl = b >> 1;
+  }
+  const int t4 = tl % nt, ky = tl / nt;
+  if (t4 * kKxTile + 2 * half >= nxh) return;  // both columns are padding (uniform)
+  const int kxl = 2 * half + c;
+  const int kx = t4 * kKxTile + kxl;
+  const bool real_col = kx < nxh;
+  const long long istride = spec_index(1, 0, 0, 0, ny, nt, 128);
+  // column kx: element (i, z) at i * istride + 4 z; this wave's half starts at z = 64 p
+  double2* col = a.spec + spec_index(0, ky, t4 * kKxTile, 64 * p, ny, nt, 128) + kxl;
+  double2* wl = lds + wv * kRegion128;
+  const double2* partner = lds + (wv ^ 1) * kRegion128;
+  double2 tw[8];
+  {
+    const int j = lane & 7;
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) tw[k2] = g.az.tw[2 * j * k2];  // exp(-2 pi i j k2 / 64)
+  }
+  const double2 t = g.az.tw[lane];  // exp(-2 pi i m / 128)
+  double2 v[kM];
+  if (real_col) {
+#pragma unroll
+    for (int h = 0; h < kM; ++h) v[h] = col[(long long)h * istride + 4 * lane];
+  }
+  z128_exchange<-1>(v, wl, partner, lane, p, t, real_col);
+  __syncthreads();  // the exchange regions are reused by the 64-point transforms
+  if (real_col) z64_fwd_regs<kM>(v, wl, lane, tw);
+  if (a.solve && real_col) {
+    const double kxv = signed_k(kx, g.nx) * a.kfac[0];
+    const double kyv = signed_k(ky, ny) * a.kfac[1];
+    const double kzv = signed_k(2 * lane + p, 128) * a.kfac[2];
+    Solver<kL>::solve(v, kxv, kyv, kzv, a.cdelta, a.sigma, a.ds, a.dslam);
+  }
+  if (real_col) {
+    z64_inv_regs<kM>(v, wl, lane, tw);
+    if (p == 1) {
+#pragma unroll
+      for (int h = 0; h < kM; ++h) v[h] = twiddle<+1>(v[h], t);
+    }
+  }
+  z128_exchange<+1>(v, wl, partner, lane, p, t, real_col);
+  if (real_col) {
+#pragma unroll
+    for (int h = 0; h < kM; ++h) col[(long long)h * istride + 4 * lane] = v[h];
+  }
+}
+
 }  // namespace per_l_<L>
 
 using namespace WLC_CAT(per_l_, WLC_L);
@@ -383,6 +491,12 @@ hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hi
     const int blocks = 2 * g.nt * g.ny;
     if (a.occ == 3) hipLaunchKernelGGL(k_zsolve64s<3>, dim3(blocks), dim3(256), 0, st, g, a);
     else hipLaunchKernelGGL(k_zsolve64s<2>, dim3(blocks), dim3(256), 0, st, g, a);
+    return hipGetLastError();
+  }
+  if (g.nz == 128 && !a.generic) {
+    // two workgroups (two columns each) per 4-kx tile
+    const int blocks = 2 * g.nt * g.ny;
+    hipLaunchKernelGGL(k_zsolve128<(kL >= 7 ? 1 : 2)>, dim3(blocks), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (g.nz == 64 && !a.generic) {
