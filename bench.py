@@ -100,6 +100,67 @@ def cpu_baseline(w, target_s=15.0):
             "single_core": {"value": n1 / dt1, "sample": f"{n1} of the same steps on 1 core"}}
 
 
+def workload_w128(n=128, seed=2):
+    """w_b = s_b (10 (sin X cos Y + sin Y cos Z + sin Z cos X) + 0.5 N(0,1)), X = 2 pi x / box,
+    one N(0,1) field from the seed, block signs s = (+1, 0, -1)."""
+    x = np.arange(n) * (2 * np.pi / n)
+    X, Y, Z = x[None, None, :], x[None, :, None], x[:, None, None]
+    base = 10.0 * (np.sin(X) * np.cos(Y) + np.sin(Y) * np.cos(Z) + np.sin(Z) * np.cos(X))
+    f = base + 0.5 * np.random.default_rng(seed).standard_normal((n, n, n))
+    return np.stack([s * f for s in (1.0, 0.0, -1.0)])
+
+
+def bench_grid128(W, args):
+    """FP64 forward + backward sweeps of the 128^3 Lmax=8 workload (349 GB of forward store
+    per sweep would not fit: checkpoints every 16 contour steps, segments recomputed during
+    the backward sweep)."""
+    n, Ns, L, lam, box, ck = 128, 256, 8, 0.5, 8.0, 16
+    Nblk = W.block_lengths(Ns, [1 / 3, 1 / 3, 1 / 3])
+    M, N, Nk = (L + 1) ** 2, n ** 3, n * n * (n // 2 + 1)
+    g = W.WormlikeMDE((n,) * 3, (box,) * 3, Nblk, (L,) * 3, (lam,) * 3, ckpt_every=ck)
+    g.set_w(workload_w128(n))
+    steps = max(1, min(args.steps, 3))
+    for _ in range(min(args.warmup, 1)):
+        g.MDE_q()
+        g.MDE_qstar()
+        g.density_device()
+    g.sync()
+    g.event_record(0)
+    Q = td = None
+    for _ in range(steps):
+        g.MDE_q()
+        g.MDE_qstar()
+        Q, td = g.density_device()
+    g.event_record(1)
+    ms = g.event_elapsed(0, 1)
+    g.sync()
+    if not (np.isfinite(Q) and np.isfinite(td)):
+        raise RuntimeError("non-finite Q/totden")
+    g.set_timing(True)
+    g.MDE_q()
+    g.MDE_qstar()
+    kms, kcnt = g.get_timing()
+    g.set_timing(False)
+    g.close()
+    contour_steps = 2 * Ns * steps
+    step_s = ms / 1e3 / contour_steps
+    b_step = algorithmic_bytes(M, N, Nk)["step"]
+    print(json.dumps({
+        "metric": "MDE contour-steps/s at 128^3 Lmax=8", "value": round(contour_steps / (ms / 1e3), 2),
+        "unit": "contour-steps/s", "n_gpus": 1, "steps": steps, "ms_per_step": round(ms / steps, 2),
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "MDE forward+backward sweep, 128^3, box 8.0, Lmax=8 (81 harmonics), "
+                               "Ns=256, lambda=0.5, w = s_b (10 (sin X cos Y + sin Y cos Z + sin Z cos X) "
+                               "+ 0.5 N(0,1) seed 2), s = (+,0,-); forward store checkpointed every 16 "
+                               "contour steps; one step = one sweep of 512 contour steps",
+                   "grid": [n, n, n], "Lmax": L, "Ns": Ns, "blocks": Nblk, "ckpt_every": ck},
+        "step_roofline": {"algorithmic_bytes": b_step, "achieved": round(b_step / step_s / 1e9, 1),
+                          "frac": round(b_step / step_s / 1e9 / PEAK_HBM_GBS, 4)},
+        "kernels_avg_us": {k: round(1e3 * m / c, 1) if c else 0.0 for k, m, c in zip(KERNELS, kms, kcnt)},
+        "Q": Q,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +174,10 @@ def main():
                          "(Lmax 4, Ns 100) through the Fortran driver -- 50 simple-mixing "
                          "iterations reported per iteration, then a simple-then-Anderson run "
                          "towards residual 1e-6 (value = its time if it converges)")
+    ap.add_argument("--grid128", action="store_true",
+                    help="instead of the 64^3 benchmark: the FP64 sweep of the 128^3 Lmax=8 "
+                         "workload (box 8, Ns 256, lambda 0.5, forward store checkpointed every "
+                         "16 contour steps), one JSON line of its own")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the counter "
                          "calibration kernel of wormlike-chain-triblock_amd/profile_summary.py "
@@ -165,6 +230,10 @@ def main():
         return
 
     import wlc_mde as W
+
+    if args.grid128:
+        bench_grid128(W, args)
+        return
 
     shape = (64, 64, 64)
     Ns = 200
