@@ -175,3 +175,19 @@ def test_fortran_restart_from_field_file(wlc, tmp_path):
     assert np.array_equal(first[:, 1:], full[:2, 1:])
     assert np.array_equal(second[:, 1:], full[2:, 1:])
     assert np.array_equal(wsecond, wfull)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran binding not built (no flang)")
+def test_split_operator_call_slot_dispatches_to_the_same_solve(wlc, tmp_path):
+    """A driver that calls split_operator_MDE_init / AB_diblock_sph_driver (the reference's
+    split-operator variant) gets the spherical-harmonic BDF solve: the trajectory and the
+    final fields equal the MDE_setup / MDE_q / MDE_qstar run bit for bit."""
+    rng = np.random.default_rng(8)
+    w = 0.5 * rng.standard_normal((3,) + CASE["shape"])
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    ref, wref, _ = run_driver(a, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05)
+    got, wgot, _ = run_driver(b, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05,
+                              flags=["--split-operator-slot"])
+    assert np.array_equal(got, ref) and np.array_equal(wgot, wref)

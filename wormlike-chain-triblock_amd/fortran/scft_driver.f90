@@ -8,6 +8,9 @@
 !            fields of input.bin (the reference's wAB_init_type < 0 -> readW,
 !            SCFT_control.f90:79-83)
 !   --dumpW  write the final fields to W0.dat (W_dump, SCFT_dump.f90:273-293)
+!   --split-operator-slot  set up and solve through the split-operator call slot
+!            (split_operator_MDE_init / AB_diblock_sph_driver) instead of MDE_setup /
+!            MDE_q / MDE_qstar (same results: the slot runs the same solve)
 ! input (stream, native endian):
 !   int32  nx, ny, nz, nblk, Ns, Nblk(3), Lmax(3), max_iter, method (0 simple mixing only,
 !          1 simple then Anderson), n_simple, n_hist, Anderson form (0 reference, 1 mixing
@@ -25,14 +28,14 @@ program scft_driver
   implicit none
   character(len=1024) :: fin, fout, fw, arg
   integer :: u, hdr(16), max_iter, method, it, a, b, n, nargs, done, i, npos, stat
-  logical :: read_w, dump_w
+  logical :: read_w, dump_w, sph_slot
   integer(8) :: c0, c1, rate
   real(8) :: box(3), lam(3), chi9(9), chi(3, 3), f(3), tol, mix(2)
   logical :: converged
 
   nargs = command_argument_count()
   fin = ''; fout = ''; fw = ''
-  read_w = .false.; dump_w = .false.
+  read_w = .false.; dump_w = .false.; sph_slot = .false.
   npos = 0
   do i = 1, nargs
      call get_command_argument(i, arg)
@@ -40,6 +43,8 @@ program scft_driver
         read_w = .true.
      else if (trim(arg) == '--dumpW') then
         dump_w = .true.
+     else if (trim(arg) == '--split-operator-slot') then
+        sph_slot = .true.
      else
         npos = npos + 1
         if (npos == 1) fin = arg
@@ -48,7 +53,8 @@ program scft_driver
      end if
   end do
   if (npos < 2 .or. npos > 3) then
-     write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin] [--readW] [--dumpW]'
+     write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin] [--readW] [--dumpW] &
+          &[--split-operator-slot]'
      error stop 2
   end if
 
@@ -69,7 +75,11 @@ program scft_driver
         chi(a, b) = chi9(3 * (a - 1) + b)
      end do
   end do
-  call MDE_setup()
+  if (sph_slot) then
+     call split_operator_MDE_init()
+  else
+     call MDE_setup()
+  end if
   n = SIDEx * SIDEy * SIDEz
   read(u) w_blk
   close(u)
@@ -91,8 +101,12 @@ program scft_driver
   done = 0
   call system_clock(c0, rate)
   do it = 1, max_iter
-     call MDE_q(.true.)
-     call MDE_qstar(.true.)
+     if (sph_slot) then
+        call AB_diblock_sph_driver()
+     else
+        call MDE_q(.true.)
+        call MDE_qstar(.true.)
+     end if
      call density()
      call free_energy(chi)
      call scft_update(it, chi, f, tol, converged)

@@ -170,6 +170,25 @@ contains
     call wlc_check(wlc_mde_backward(handle), 'wlc_mde_backward')
   end subroutine MDE_qstar
 
+  ! Call slot of the reference's split-operator MDE (split_operator_MDE_mod.f90:24-115
+  ! split_operator_MDE_init, called whenever the cell changes; :120-238
+  ! AB_diblock_sph_driver, both propagators).  A driver built for that variant keeps its
+  ! calls; here they run the spherical-harmonic BDF solve of MDE_q / MDE_qstar (the
+  ! split-operator numerics on the (theta, phi) grid are not rebuilt), and density
+  ! consumes the result as after MDE_q / MDE_qstar.
+  subroutine split_operator_MDE_init()
+    if (c_associated(handle)) then
+       call MDE_set_box(box_x, box_y, box_z)
+    else
+       call MDE_setup()
+    end if
+  end subroutine split_operator_MDE_init
+
+  subroutine AB_diblock_sph_driver()
+    call MDE_q(.true.)
+    call MDE_qstar(.true.)
+  end subroutine AB_diblock_sph_driver
+
   ! Q, totden and phi per block (SCFT_Density_FE.f90:3-159)
   subroutine density()
     real(c_double) :: Q, td
