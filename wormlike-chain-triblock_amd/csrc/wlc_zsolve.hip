@@ -386,22 +386,26 @@ constexpr int kRegion128 = kRegion > kXB * 64 ? kRegion : kXB * 64;
 
 // exchange with the partner wave of the column: own values in, partner's values out,
 // kXB harmonics at a time; every wave of the workgroup takes part in the barriers
-template <int DIR>
-__device__ __forceinline__ void z128_exchange(double2 (&v)[kM], double2* own, const double2* other,
+// NLOOP >= NH sets the number of batches, so that waves exchanging different numbers of
+// harmonics in one workgroup still pass the same number of barriers
+template <int DIR, int NH, int NLOOP = NH, int XB = kXB, int N>
+__device__ __forceinline__ void z128_exchange(double2 (&v)[N], double2* own, const double2* other,
                                               int lane, int p, double2 t, bool real_col) {
+  static_assert(XB <= kXB, "exchange batch fits a wave region");
+  static_assert(NLOOP >= NH, "every harmonic is exchanged");
 #pragma unroll
-  for (int b0 = 0; b0 < kM; b0 += kXB) {
+  for (int b0 = 0; b0 < NLOOP; b0 += XB) {
     __syncthreads();  // the partner has read the previous batch
     if (real_col) {
 #pragma unroll
-      for (int q = 0; q < kXB; ++q)
-        if (b0 + q < kM) own[q * 64 + lane] = v[b0 + q];
+      for (int q = 0; q < XB; ++q)
+        if (b0 + q < NH) own[q * 64 + lane] = v[b0 + q];
     }
     __syncthreads();
     if (real_col) {
 #pragma unroll
-      for (int q = 0; q < kXB; ++q) {
-        if (b0 + q < kM) {
+      for (int q = 0; q < XB; ++q) {
+        if (b0 + q < NH) {
           const double2 o = other[q * 64 + lane], x = v[b0 + q];
           if (DIR < 0) {
             // forward: wave 0 keeps x(m) + x(m + 64), wave 1 (x(m) - x(m + 64)) w^m
@@ -457,7 +461,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
 #pragma unroll
     for (int h = 0; h < kM; ++h) v[h] = col[(long long)h * istride + 4 * lane];
   }
-  z128_exchange<-1>(v, wl, partner, lane, p, t, real_col);
+  z128_exchange<-1, kM>(v, wl, partner, lane, p, t, real_col);
   __syncthreads();  // the exchange regions are reused by the 64-point transforms
   if (real_col) z64_fwd_regs<kM>(v, wl, lane, tw);
   if (a.solve && real_col) {
@@ -473,10 +477,120 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
       for (int h = 0; h < kM; ++h) v[h] = twiddle<+1>(v[h], t);
     }
   }
-  z128_exchange<+1>(v, wl, partner, lane, p, t, real_col);
+  z128_exchange<+1, kM>(v, wl, partner, lane, p, t, real_col);
   if (real_col) {
 #pragma unroll
     for (int h = 0; h < kM; ++h) col[(long long)h * istride + 4 * lane] = v[h];
+  }
+}
+
+// nz = 128 with the degree split of k_zsolve64s: workgroup = one column, four waves
+// wv = 2 part + p (part 0: l <= kLS, part 1: l > kLS; p: z half).  The radix-2 step pairs
+// wv with wv ^ 1, the crossing tridiagonal chains pair wv with wv ^ 2 through the region of
+// the part-0 wave of the same z half.  Each wave holds about half the harmonics, so two
+// waves fit per SIMD at Lmax 7 and 8.
+template <int OCC>
+__global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
+  __shared__ double2 lds[4 * kRegion128];
+  static_assert(kXch + 2 <= kRegion128, "crossing chains fit a wave region");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = wv & 1, part = wv >> 1;
+  const int ny = g.ny, nxh = g.nxh, nt = g.nt;
+  // block -> column: the four columns of a 4-kx tile run on one XCD (blocks b, b + 8,
+  // b + 16, b + 24), so each tile's cache lines are fetched once
+  const int nb = gridDim.x, b = blockIdx.x;
+  int tl, kxl;
+  if ((nb & 31) == 0) {
+    const int t = b >> 3;
+    kxl = t & 3;
+    tl = (t >> 2) * 8 + (b & 7);
+  } else {
+    kxl = b & 3;
+    tl = b >> 2;
+  }
+  const int t4 = tl % nt, ky = tl / nt;
+  const int kx = t4 * kKxTile + kxl;
+  if (kx >= nxh) return;  // a padding column (uniform over the workgroup)
+  const long long istride = spec_index(1, 0, 0, 0, ny, nt, 128);
+  double2* col = a.spec + spec_index(0, ky, t4 * kKxTile, 64 * p, ny, nt, 128) + kxl;
+  double2* wl = lds + wv * kRegion128;
+  const double2* partner = lds + (wv ^ 1) * kRegion128;
+  double2* xch = lds + p * kRegion128;  // region of the part-0 wave of this z half
+  // the twiddles are reloaded (cache hits) around each transform instead of being held
+  // through the solve, where the registers are needed for the harmonics
+  const double2* twz = g.az.tw;  // exp(-2 pi i m / 128); exp(-2 pi i j k2 / 64) = twz[2 j k2]
+  const bool active = a.solve;
+  const KFrame f(signed_k(kx, g.nx) * a.kfac[0], signed_k(ky, ny) * a.kfac[1],
+                 signed_k(2 * lane + p, 128) * a.kfac[2], a.sigma, a.ds);
+  double2 v[kNV];
+  {
+    const double2 t = g.az.tw[lane];  // exp(-2 pi i m / 128)
+    if (part == 0) {
+#pragma unroll
+      for (int h = 0; h < PartLo::NH; ++h) v[h] = col[(long long)(PartLo::B0 + h) * istride + 4 * lane];
+      z128_exchange<-1, PartLo::NH, kNV>(v, wl, partner, lane, p, t, true);
+    } else {
+#pragma unroll
+      for (int h = 0; h < PartHi::NH; ++h) v[h] = col[(long long)(PartHi::B0 + h) * istride + 4 * lane];
+      z128_exchange<-1, PartHi::NH, kNV>(v, wl, partner, lane, p, t, true);
+    }
+  }
+  __syncthreads();  // the exchange regions are reused by the transforms and the chains
+  {
+    double2 tw[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) tw[k2] = twz[2 * (lane & 7) * k2];
+    if (part == 0) {
+      z64_fwd_regs<PartLo::NH>(v, wl, lane, tw);
+    } else {
+      z64_fwd_regs<PartHi::NH>(v, wl, lane, tw);
+    }
+  }
+  // xch is the part-0 wave's own region: its transform is done (program order), and the
+  // part-1 wave reads xch only after the next barrier
+  if (part == 0) {
+    if (active) {
+      PartLo::to_frame(v, f);
+      tri_lower_forward<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+    }
+  } else {
+    if (active) PartHi::to_frame(v, f);
+  }
+  __syncthreads();
+  if (part == 1 && active) tri_upper<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+  __syncthreads();
+  if (part == 0) {
+    if (active) {
+      tri_lower_back<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+      PartLo::from_frame(v, f);
+    }
+  } else {
+    if (active) PartHi::from_frame(v, f);
+  }
+  // after the last barrier only the part-0 wave reads xch (its own region), so the
+  // transforms below need no further barrier
+  double2 tw[8];
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) tw[k2] = twz[2 * (lane & 7) * k2];
+  const double2 t = g.az.tw[lane];
+  if (part == 0) {
+    z64_inv_regs<PartLo::NH>(v, wl, lane, tw);
+    if (p == 1) {
+#pragma unroll
+      for (int h = 0; h < PartLo::NH; ++h) v[h] = twiddle<+1>(v[h], t);
+    }
+    z128_exchange<+1, PartLo::NH, kNV>(v, wl, partner, lane, p, t, true);
+#pragma unroll
+    for (int h = 0; h < PartLo::NH; ++h) col[(long long)(PartLo::B0 + h) * istride + 4 * lane] = v[h];
+  } else {
+    z64_inv_regs<PartHi::NH>(v, wl, lane, tw);
+    if (p == 1) {
+#pragma unroll
+      for (int h = 0; h < PartHi::NH; ++h) v[h] = twiddle<+1>(v[h], t);
+    }
+    z128_exchange<+1, PartHi::NH, kNV>(v, wl, partner, lane, p, t, true);
+#pragma unroll
+    for (int h = 0; h < PartHi::NH; ++h) col[(long long)(PartHi::B0 + h) * istride + 4 * lane] = v[h];
   }
 }
 
@@ -491,6 +605,12 @@ hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hi
     const int blocks = 2 * g.nt * g.ny;
     if (a.occ == 3) hipLaunchKernelGGL(k_zsolve64s<3>, dim3(blocks), dim3(256), 0, st, g, a);
     else hipLaunchKernelGGL(k_zsolve64s<2>, dim3(blocks), dim3(256), 0, st, g, a);
+    return hipGetLastError();
+  }
+  if (g.nz == 128 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
+    // one workgroup (one column, harmonics split over two waves per z half) per column
+    const int blocks = 4 * g.nt * g.ny;
+    hipLaunchKernelGGL(k_zsolve128s<2>, dim3(blocks), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (g.nz == 128 && !a.generic) {
