@@ -136,11 +136,12 @@ def bench_grid128(W, args):
     g.sync()
     if not (np.isfinite(Q) and np.isfinite(td)):
         raise RuntimeError("non-finite Q/totden")
-    g.set_timing(True)
+    g.set_timing(2)
     g.MDE_q()
     g.MDE_qstar()
+    g.density_device()
     kms, kcnt = g.get_timing()
-    g.set_timing(False)
+    g.set_timing(0)
     g.close()
     contour_steps = 2 * Ns * steps
     step_s = ms / 1e3 / contour_steps
@@ -267,12 +268,17 @@ def main():
     contour_steps = 2 * Ns * args.steps
     value = contour_steps / (ms / 1e3)
 
-    # per-kernel average launch durations (HIP events on the handle's stream)
-    g.set_timing(True)
+    # Per-kernel launch durations: one more sweep pair with HIP events around every launch
+    # on the stream it runs on.  In the timed sweeps above the forward and backward sweeps
+    # run concurrently, so a kernel there shares the GPU with the other sweep's kernels;
+    # here (timing mode 2) the backward sweep waits for the forward one, so each step
+    # kernel has the GPU to itself and bytes / duration is that kernel's own rate.
+    g.set_timing(2)
     g.MDE_q()
     g.MDE_qstar()
+    g.density_device()
     kms, kcnt = g.get_timing()
-    g.set_timing(False)
+    g.set_timing(0)
     avg_us = {k: (1e3 * m / c if c else 0.0) for k, m, c in zip(KERNELS, kms, kcnt)}
     ab = algorithmic_bytes(M, N, Nk)
     dom = max(KERNELS, key=lambda k: kms[KERNELS.index(k)])

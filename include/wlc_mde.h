@@ -108,8 +108,12 @@ int wlc_mde_sync(wlc_handle* h);
 /* the HIP stream (hipStream_t) all work of the handle is queued on */
 void* wlc_mde_stream(wlc_handle* h);
 
-/* Per-kernel timing with HIP events on the handle's stream.  While enabled, every
- * launch of the step kernels is bracketed by events; wlc_mde_get_timing returns,
+/* Per-kernel timing with HIP events.  enable: 0 off; 1 every launch of the step kernels
+ * is bracketed by events on the stream it runs on, read back when the host waits anyway
+ * (wlc_mde_density*, wlc_mde_get_timing, wlc_mde_set_timing), so the sweeps keep running
+ * concurrently and a duration includes the time a kernel shares the GPU; 2 as 1, but
+ * wlc_mde_forward waits for its sweep, so each step kernel is timed with the GPU to
+ * itself (the density terms still overlap the backward sweep).  wlc_mde_get_timing returns,
  * for kernel k in {0: rhs+xy FFT, 1: z FFT+solve, 2: inverse xy FFT, 3: density
  * accumulation, 4: inverse xy FFT of one step fused with rhs+xy FFT of the next}, the
  * summed milliseconds and the launch count since enabling.

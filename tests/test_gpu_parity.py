@@ -261,3 +261,37 @@ def test_checkpointed_forward_store_matches_full_store(wlc, monkeypatch, K):
     (p0, Q0, t0, q0, s0), (p1, Q1, t1, q1, s1) = res
     assert np.array_equal(q0, q1) and np.array_equal(s0, s1)
     assert Q0 == Q1 and t0 == t1 and np.array_equal(p0, p1)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_timing_counts_launches_without_changing_results(wlc, mode):
+    """Per-kernel timing brackets every launch with events on its own stream (mode 1: read
+    back at the density call; mode 2: also finishing the forward sweep before the backward
+    one): the launch counts follow the sweep structure and the results are bitwise those of
+    an untimed run."""
+    shape, box, Nblk, L, lam = (8, 16, 16), (2.0, 3.0, 3.0), (4, 5, 3), (2, 3, 1), (1.0, 0.6, 1.5)
+    rng = np.random.default_rng(41)
+    w = rng.standard_normal((3,) + shape)
+    g = wlc.WormlikeMDE(shape, box, Nblk, L, lam)
+    g.set_w(w)
+    g.MDE_q()
+    g.MDE_qstar()
+    phi0, Q0, td0 = g.density()
+    g.set_timing(mode)
+    for _ in range(2):
+        g.MDE_q()
+        g.MDE_qstar()
+        phi1, Q1, td1 = g.density()
+    ms, cnt = g.get_timing()
+    g.set_timing(0)
+    with pytest.raises(wlc.WlcError):
+        g.set_timing(3)
+    g.close()
+    assert np.array_equal(phi1, phi0) and Q1 == Q0 and td1 == td0
+    Ns = sum(Nblk)
+    # per sweep pair: Ns z solves and one A, Ns - 1 fused C+A, one C per direction;
+    # Ns + 1 density terms
+    assert cnt[1] == 2 * 2 * Ns
+    assert cnt[0] == 2 * 2 and cnt[2] == 2 * 2 and cnt[4] == 2 * 2 * (Ns - 1)
+    assert cnt[3] == 2 * (Ns + 1)
+    assert all(m > 0.0 for m in ms)

@@ -139,6 +139,7 @@ struct wlc_handle {
   int fuse = 1;  // kernel C of a step fused with kernel A of the next
   // timing
   bool timing = false;
+  bool timing_serial = false;  // timing mode 2: the backward sweep waits for the forward
   struct Rec { int k; hipEvent_t a, b; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
@@ -645,7 +646,6 @@ int backward_concurrent(wlc_handle* h) {
   if (rc) return rc;
   if (dens) HIPCHK(wlc::launch_axpy(h->rho, 1.0, h->rho2, nacc, h->st));
   h->bwd_done = dens;
-  if (h->timing) return flush_timing(h);
   return WLC_OK;
 }
 
@@ -872,6 +872,11 @@ int wlc_mde_forward(wlc_handle* h) {
   if (!h) return fail(WLC_EINVAL, "NULL handle");
   if (!h->have_w) return fail(WLC_ESTATE, "set_w must precede forward");
   HIPCHK(hipSetDevice(h->device));
+  // bound the pending timing records of callers that never read them back
+  if (h->timing && h->recs.size() > 16384) {
+    const int rt = flush_timing(h);
+    if (rt) return rt;
+  }
   if (h->conc) {
     HIPCHK(hipEventRecord(h->ev_pre, h->st));  // the backward sweep may start from here
     h->pre_recorded = true;
@@ -883,7 +888,9 @@ int wlc_mde_forward(wlc_handle* h) {
   if (rc) return rc;
   h->fwd_done = true;
   h->bwd_done = false;
-  if (h->timing) return flush_timing(h);
+  // timing mode 2: finish the forward sweep before anything else is queued, so that each
+  // step kernel is timed with the GPU to itself
+  if (h->timing_serial) return flush_timing(h);
   return WLC_OK;
 }
 
@@ -917,7 +924,6 @@ int wlc_mde_backward(wlc_handle* h) {
   if (rc) return rc;
   if (rj) return rj;
   h->bwd_done = dens;
-  if (h->timing) return flush_timing(h);
   return WLC_OK;
 }
 
@@ -932,6 +938,12 @@ static int density_common(wlc_handle* h, double* Q, double* totden) {
   double sc[2];
   HIPCHK(hipMemcpyAsync(sc, h->scal, sizeof(sc), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  // timed launches are read out here, where the host waits anyway, so that timing does not
+  // add a synchronisation between the forward and the backward sweep
+  if (h->timing) {
+    const int rt = flush_timing(h);
+    if (rt) return rt;
+  }
   const double Qv = sc[0] / V;
   if (!std::isfinite(Qv) || !std::isfinite(sc[1]) || sc[1] == 0.0) return fail(WLC_ENONFINITE, "Q or totden not finite");
   if (Q) *Q = Qv;
@@ -1002,7 +1014,9 @@ int wlc_mde_set_timing(wlc_handle* h, int enable) {
   if (!h) return fail(WLC_EINVAL, "NULL handle");
   int rc = flush_timing(h);
   if (rc) return rc;
+  if (enable < 0 || enable > 2) return fail(WLC_EINVAL, "timing mode must be 0, 1 or 2");
   h->timing = enable != 0;
+  h->timing_serial = enable == 2;
   for (int k = 0; k < 5; ++k) { h->ms[k] = 0.0; h->cnt[k] = 0; }
   return WLC_OK;
 }

@@ -2,6 +2,9 @@
 
     python wormlike-chain-triblock_amd/profile_summary.py kernels <kernel_stats.csv>
         markdown table of per-kernel calls / average / min / max / share
+    python wormlike-chain-triblock_amd/profile_summary.py last <kernel_trace.csv> <pairs>
+        per bench kernel, the average duration of its launches in the last of <pairs> sweep
+        pairs (bench.py's per-kernel timing pass), to compare with the bench line
     python wormlike-chain-triblock_amd/profile_summary.py traffic <fetch.csv> <write.csv> \
             <copy_fetch.csv> <copy_write.csv>
         HBM bytes per launch from FETCH_SIZE and WRITE_SIZE counter runs of bench.py (KiB
@@ -17,9 +20,9 @@ import sys
 
 # bench.py kernel keys -> substrings of the HIP kernel names
 KEYS = [
-    ("inv_rhs_fft_xy", ("k_inv_rhs64", "k_inv_rhs<")),
-    ("rhs_fft_xy", ("k_rhs64", "k_rhs_fft_xy")),
-    ("ifft_xy", ("k_inv64", "k_ifft_xy")),
+    ("inv_rhs_fft_xy", ("k_inv_rhs64", "k_inv_rhs128", "k_inv_rhs<")),
+    ("rhs_fft_xy", ("k_rhs64", "k_rhs128", "k_rhs_fft_xy")),
+    ("ifft_xy", ("k_inv64", "k_inv128", "k_ifft_xy")),
     ("zsolve", ("k_zsolve",)),
     ("density_acc", ("k_density_acc",)),
 ]
@@ -31,6 +34,22 @@ def key_of(name):
         if any(p in name for p in pats):
             return key
     return None
+
+
+def last(path, pairs):
+    """Average launch duration per bench kernel over its last len / pairs launches."""
+    per = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        k = key_of(r["Kernel_Name"])
+        if k:
+            per[k].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    print("# kernel            launches_in_last_pair  average_us")
+    for k in sorted(per):
+        v = sorted(per[k])
+        n = len(v) // pairs
+        tail = v[len(v) - n:]
+        avg = sum(b - a for a, b in tail) / max(n, 1) / 1e3
+        print(f"# {k:<18} {n:>10}  {avg:10.2f}")
 
 
 def kernels(path):
@@ -93,6 +112,8 @@ def traffic(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv):
 if __name__ == "__main__":
     if len(sys.argv) >= 3 and sys.argv[1] == "kernels":
         kernels(sys.argv[2])
+    elif len(sys.argv) >= 4 and sys.argv[1] == "last":
+        last(sys.argv[2], int(sys.argv[3]))
     elif len(sys.argv) >= 6 and sys.argv[1] == "traffic":
         traffic(*sys.argv[2:6])
     else:

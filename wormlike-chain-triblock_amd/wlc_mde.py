@@ -203,7 +203,8 @@ class WormlikeMDE:
 
     # --- measurement ---------------------------------------------------------------
     def set_timing(self, enable):
-        _check(_lib.wlc_mde_set_timing(self.h, 1 if enable else 0))
+        """0/False off, 1/True live (sweeps stay concurrent), 2 sweeps serialized."""
+        _check(_lib.wlc_mde_set_timing(self.h, int(enable)))
 
     def get_timing(self):
         ms = (ctypes.c_double * 5)()
