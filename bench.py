@@ -162,6 +162,71 @@ def bench_grid128(W, args):
     }))
 
 
+def free_energy(w, phi, Q, chi, box):
+    """F = -ln Q + (1/V) int [sum_{a<b} chi_ab phi_a phi_b - sum_a w_a phi_a - xi (1 - sum phi)],
+    xi = mean of the block fields, with the periodic-Simpson volume weights (the ABC form
+    of SCFT_Density_FE.f90:296-298,333)."""
+    nz, ny, nx = phi.shape[1:]
+    c = [np.where(np.arange(n) % 2 == 0, 2.0 / 3.0, 4.0 / 3.0) for n in (nz, ny, nx)]
+    wgt = c[0][:, None, None] * c[1][None, :, None] * c[2][None, None, :] / (nx * ny * nz)
+    nb = len(phi)
+    xi = w.mean(0)
+    e = -xi * (1.0 - phi.sum(0))
+    for a in range(nb):
+        e -= w[a] * phi[a]
+        for b in range(a + 1, nb):
+            e += chi[a][b] * phi[a] * phi[b]
+    return -np.log(Q) + float((wgt * e).sum())
+
+
+def bench_spectrum_fp32(W, args):
+    """The 64^3 Lmax=6 sweep with the half spectrum stored as FP32: contour-steps/s, and
+    the differences of phi, Q and F (chiN = 20 for every pair) from an FP64 run."""
+    shape, Ns, L, lam, box = (64, 64, 64), 200, 6, 1.0, 5.0
+    Nblk = W.block_lengths(Ns, [1 / 3, 1 / 3, 1 / 3])
+    w = workload_w(shape)
+    chi = np.full((3, 3), 20.0) - 20.0 * np.eye(3)
+    out = {}
+    for fp32 in (False, True):
+        g = W.WormlikeMDE(shape, (box,) * 3, Nblk, (L,) * 3, (lam,) * 3, spectrum_fp32=fp32)
+        g.set_w(w)
+        for _ in range(args.warmup):
+            g.MDE_q()
+            g.MDE_qstar()
+            g.density_device()
+        g.sync()
+        g.event_record(0)
+        for _ in range(args.steps):
+            g.MDE_q()
+            g.MDE_qstar()
+            g.density_device()
+        g.event_record(1)
+        ms = g.event_elapsed(0, 1)
+        g.MDE_q()
+        g.MDE_qstar()
+        phi, Q, _ = g.density()
+        g.close()
+        out[fp32] = (2 * Ns * args.steps / (ms / 1e3), phi, Q, free_energy(w, phi, Q, chi, box))
+    v64, phi64, Q64, F64 = out[False]
+    v32, phi32, Q32, F32 = out[True]
+    step_bytes = 32 * 49 * 64 ** 3 + 8 * 64 ** 3 + 32 * 49 * 64 * 64 * 33
+    print(json.dumps({
+        "metric": "MDE contour-steps/s at 64^3 Lmax=6, FP32 half spectrum", "value": round(v32, 1),
+        "unit": "contour-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "dtype": "f64 arithmetic, f32 spectrum storage",
+        "data": "synthetic", "fp64_value": round(v64, 1),
+        "config": {"workload": "the 64^3 benchmark sweep (box 5.0, Lmax=6, Ns=200, lambda=1.0, "
+                               "smoothed N(0,1) w seed 1, std 5); the half spectrum between the "
+                               "FFT passes and the solve stored as FP32", "grid": [64, 64, 64],
+                   "Lmax": L, "Ns": Ns},
+        "step_roofline": {"algorithmic_bytes": step_bytes,
+                          "frac": round(step_bytes * v32 / 1e9 / PEAK_HBM_GBS, 4)},
+        "error_vs_fp64": {"phi_max_rel": float(np.abs(phi32 - phi64).max() / np.abs(phi64).max()),
+                          "Q_rel": abs(Q32 - Q64) / Q64, "F_abs_kT": abs(F32 - F64), "F_fp64": F64,
+                          "chiN": 20.0},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +244,10 @@ def main():
                     help="instead of the 64^3 benchmark: the FP64 sweep of the 128^3 Lmax=8 "
                          "workload (box 8, Ns 256, lambda 0.5, forward store checkpointed every "
                          "16 contour steps), one JSON line of its own")
+    ap.add_argument("--spectrum-fp32", action="store_true",
+                    help="the 64^3 sweep with the half spectrum stored as FP32 (all arithmetic "
+                         "FP64), one JSON line of its own with the phi / Q / F differences "
+                         "from an FP64 run on the same fields")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the counter "
                          "calibration kernel of wormlike-chain-triblock_amd/profile_summary.py "
@@ -234,6 +303,9 @@ def main():
 
     if args.grid128:
         bench_grid128(W, args)
+        return
+    if args.spectrum_fp32:
+        bench_spectrum_fp32(W, args)
         return
 
     shape = (64, 64, 64)

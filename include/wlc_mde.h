@@ -52,7 +52,12 @@ typedef struct wlc_params {
                                      checkpoints every K steps, segments recomputed during
                                      the backward sweep (no counterpart in the reference,
                                      which stores every slice, init_array_2dcomp_mod.f90:47-51) */
-  int reserved[6];                /* must be zero */
+  int spectrum_fp32;              /* 1: store the half spectrum (the FFT / solve intermediate)
+                                     as FP32, every arithmetic step stays FP64; halves the
+                                     spectrum traffic at ~1e-7 rounding per store.  No
+                                     counterpart in the reference beyond its own FP32 staging
+                                     of the G-applied spectrum (wormlike_MDE_2dcomp_mod.f90:429) */
+  int reserved[5];                /* must be zero */
 } wlc_params;
 
 typedef struct wlc_handle wlc_handle;

@@ -103,3 +103,29 @@ def test_config2_reversal_determinism_and_shift(c2):
     phis, Qs, _ = c2.density()
     assert abs(Qs - Q) / Q < 1e-12
     assert _rel(phis, np.roll(phi, (2, -4, 6), axis=(1, 2, 3))) < 1e-11
+
+
+def test_fp32_spectrum_at_the_benchmark_size(wlc):
+    """64^3 / Lmax 6 / Ns 200 with the FP32 half spectrum against the FP64 path on the same
+    smoothed fields (std 5): stated tolerances phi 5e-6, Q 1e-7 relative (observed: phi 7e-7
+    on the benchmark fields; Q 2.4e-10 there and 1.5e-8 on these)."""
+    rng = np.random.default_rng(1)
+    w64 = rng.standard_normal((3, 64, 64, 64))
+    k = 2 * np.pi * np.fft.fftfreq(64)
+    filt = np.exp(-2.0 * (k[:, None, None] ** 2 + k[None, :, None] ** 2 +
+                          np.fft.rfftfreq(64)[None, None, :] ** 2 * (2 * np.pi) ** 2))
+    for b in range(3):
+        f = np.fft.irfftn(np.fft.rfftn(w64[b]) * filt, s=(64, 64, 64), axes=(0, 1, 2))
+        w64[b] = 5.0 * (f - f.mean()) / f.std()
+    res = []
+    for fp32 in (False, True):
+        g = wlc.WormlikeMDE((64, 64, 64), (5.0,) * 3, (67, 67, 66), (6, 6, 6), (1.0,) * 3,
+                            spectrum_fp32=fp32)
+        g.set_w(w64)
+        g.MDE_q()
+        g.MDE_qstar()
+        res.append(g.density()[:2])
+        g.close()
+    (p64, Q64), (p32, Q32) = res
+    assert np.abs(p32 - p64).max() / np.abs(p64).max() < 5e-6
+    assert abs(Q32 - Q64) / Q64 < 1e-7

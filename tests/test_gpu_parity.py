@@ -295,3 +295,42 @@ def test_timing_counts_launches_without_changing_results(wlc, mode):
     assert cnt[0] == 2 * 2 and cnt[2] == 2 * 2 and cnt[4] == 2 * 2 * (Ns - 1)
     assert cnt[3] == 2 * (Ns + 1)
     assert all(m > 0.0 for m in ms)
+
+
+# FP32 half spectrum (all arithmetic FP64, the spectrum rounded to FP32 where it is stored,
+# twice per contour step): stated tolerance against the FP64 oracle 2e-6 relative in phi,
+# Q and q(Ns) on these short chains (observed 1e-8 .. 3e-7).
+FP32_TOL = 2e-6
+FP32_CASES = [
+    ((64, 4, 6), (3.0, 2.0, 2.5), (4, 3, 4), (6, 5, 7), (1.0, 0.7, 1.3)),     # split z, nz 64
+    ((4, 64, 64), (2.0, 5.0, 4.0), (3, 4, 3), (3, 2, 3), (1.0, 0.5, 2.0)),    # 64x64 planes
+    ((2, 128, 128), (1.0, 4.0, 5.0), (2, 3, 2), (3, 2, 3), (1.0, 0.5, 2.0)),  # 128x128 planes
+    ((128, 4, 6), (6.0, 2.0, 2.5), (3, 4), (8, 7), (0.5, 1.0)),               # nz 128, both kernels
+    ((6, 12, 6), (2.0, 3.0, 2.5), (3, 5, 4), (1, 3, 2), (2.0, 0.3, 1.0)),     # generic kernels
+    ((8, 8, 8), (4.0, 4.0, 4.0), (9,), (4,), (0.7,)),                         # one-wave z, nz 8
+]
+
+
+@pytest.mark.parametrize("case", FP32_CASES, ids=[f"{c[0]}-L{c[3]}" for c in FP32_CASES])
+def test_fp32_spectrum_matches_oracle_within_stated_tolerance(wlc, oracle, case):
+    shape, box, Nblk, L, lam = case
+    rng = np.random.default_rng(sum(shape))
+    w = 1.5 * rng.standard_normal((len(Nblk),) + shape)
+    orc = oracle.Oracle(shape, box, Nblk, L, lam)
+    q = orc.forward(w)
+    qs = orc.backward(w)
+    phi, Q, td = orc.density(q, qs)
+    out = []
+    for fp32 in (False, True):
+        g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, spectrum_fp32=fp32)
+        g.set_w(w)
+        g.MDE_q()
+        g.MDE_qstar()
+        gphi, gQ, _ = g.density()
+        out.append((gphi, gQ, g.get_q(+1, sum(Nblk))))
+        g.close()
+    (p64, Q64, q64), (p32, Q32, q32) = out
+    assert _rel(p32, phi) < FP32_TOL and abs(Q32 - Q) / Q < FP32_TOL
+    assert _rel(q32, orc.slice(q, sum(Nblk))) < FP32_TOL
+    # the mode really stores FP32: the result differs from the FP64 path
+    assert _rel(p32, p64) > 1e-12

@@ -75,11 +75,23 @@ def test_checkpoint_interval_is_validated(wlc, k):
     assert b"ckpt_every" in lib.wlc_mde_last_error()
 
 
+@pytest.mark.parametrize("v", [-1, 2])
+def test_spectrum_precision_flag_is_validated(wlc, v):
+    lib = wlc.load()
+    p = _params(wlc)
+    p.spectrum_fp32 = v
+    h = ctypes.c_void_p()
+    assert lib.wlc_mde_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"spectrum_fp32" in lib.wlc_mde_last_error()
+
+
 def test_params_layout_matches_the_header(wlc):
     # int nx,ny,nz; double box[3]; int nblk,Ns,Nblk[3],Lmax[3]; double lambda[3];
-    # int keep_qstar, ckpt_every, reserved[6]
+    # int keep_qstar, ckpt_every, spectrum_fp32, reserved[5]
     assert ctypes.sizeof(wlc.WlcParams) == 4 * 3 + 4 + 8 * 3 + 4 * 8 + 8 * 3 + 4 * 8
     assert wlc.WlcParams.ckpt_every.offset == wlc.WlcParams.keep_qstar.offset + 4
+    assert wlc.WlcParams.spectrum_fp32.offset == wlc.WlcParams.ckpt_every.offset + 4
+    assert wlc.WlcParams.reserved.offset == wlc.WlcParams.spectrum_fp32.offset + 4
 
 
 def test_null_arguments_are_errors(wlc):

@@ -21,9 +21,23 @@ __host__ __device__ inline long long spec_index(int i, int ky, int kx, int z, in
   return ((((long long)i * ny + ky) * nt + (kx >> 2)) * nz + z) * kKxTile + (kx & 3);
 }
 
+// The half spectrum is stored as double2 (FP64) or, in the FP32-spectrum mode, as float2;
+// every kernel computes in FP64 and converts where it loads or stores the spectrum
+// (round to nearest on the store).
+__device__ __forceinline__ double2 spec_ld(const double2* p) { return *p; }
+__device__ __forceinline__ double2 spec_ld(const float2* p) {
+  const float2 v = *p;
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ __forceinline__ void spec_st(double2* p, double2 v) { *p = v; }
+__device__ __forceinline__ void spec_st(float2* p, double2 v) {
+  *p = make_float2((float)v.x, (float)v.y);
+}
+
 struct Geom {
   int nx, ny, nz, nxh;
   int nt;          // kx tiles
+  int spec32;      // 1: the half spectrum is stored as float2
   long long zs;    // real-space z-plane stride (doubles): nx*ny + pad
   long long hs;    // real-space field / harmonic stride (doubles): nz*zs + pad
   AxisDev ax, ay, az;

@@ -599,6 +599,8 @@ int validate(const wlc_params* p) {
   if (p->Ns < 3) return fail(WLC_EINVAL, "Ns must be >= 3");
   if (p->ckpt_every < 0 || (p->ckpt_every > 0 && p->ckpt_every < 3))
     return fail(WLC_EINVAL, "ckpt_every must be 0 (automatic) or >= 3");
+  if (p->spectrum_fp32 != 0 && p->spectrum_fp32 != 1)
+    return fail(WLC_EINVAL, "spectrum_fp32 must be 0 or 1");
   for (int r : p->reserved)
     if (r) return fail(WLC_EINVAL, "reserved fields must be zero");
   return WLC_OK;
@@ -737,7 +739,8 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     return fail(WLC_EHIP, "event creation failed");
   }
   const size_t store_bytes = sizeof(double) * (size_t)h->off[h->Ns + 1];
-  const size_t spec_bytes = sizeof(double2) * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz;
+  const size_t spec_elem = p->spectrum_fp32 ? sizeof(float2) : sizeof(double2);
+  const size_t spec_bytes = spec_elem * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz;
   const size_t slot_bytes = sizeof(double) * (size_t)slot_size(h);
   const size_t field_bytes = sizeof(double) * (size_t)p->nblk * h->hs;
   {
@@ -824,6 +827,7 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   wlc::Geom& g = h->geom;
   g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nxh = h->nxh;
   g.nt = wlc::kx_tiles(h->nxh);
+  g.spec32 = p->spectrum_fp32;
   g.zs = h->zs;
   g.hs = h->hs;
   wlc::AxisDev* ax[3] = {&g.ax, &g.ay, &g.az};

@@ -76,8 +76,8 @@ __device__ __forceinline__ int pk_at(int y, int x, int nx) {
 
 // packed real plane in LDS -> x pass (two real rows per complex transform), unpack,
 // y pass -> half-spectrum plane (i, z)
-template <int NX, int NY, int MAXE>
-__device__ __forceinline__ void forward_xy(double2* lds, const Geom& g, double2* __restrict__ spec,
+template <int NX, int NY, int MAXE, class S>
+__device__ __forceinline__ void forward_xy(double2* lds, const Geom& g, S* __restrict__ spec,
                                           int i, int z) {
   const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
   const int nxh = nx / 2 + 1;
@@ -116,15 +116,15 @@ __device__ __forceinline__ void forward_xy(double2* lds, const Geom& g, double2*
   // half spectrum, 64-byte runs of 4 kx per (ky, tile, z)
   for (int e = threadIdx.x; e < ny * nxh; e += kThreads) {
     const int ky = e / nxh, kx = e - ky * nxh;
-    spec[spec_index(i, ky, kx, z, ny, g.nt, g.nz)] = lds[lp<NX>(e)];
+    spec_st(spec + spec_index(i, ky, kx, z, ny, g.nt, g.nz), lds[lp<NX>(e)]);
   }
 }
 
 // half-spectrum plane (i, z) -> y pass, Hermitian pack, x pass -> packed real plane
 // in LDS (unscaled)
-template <int NX, int NY, int MAXE>
+template <int NX, int NY, int MAXE, class S>
 __device__ __forceinline__ void inverse_xy(double2* lds, const Geom& g,
-                                          const double2* __restrict__ spec, int i, int z) {
+                                          const S* __restrict__ spec, int i, int z) {
   const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
   const int nxh = nx / 2 + 1;
   const int tin = ny * nxh;
@@ -134,7 +134,8 @@ __device__ __forceinline__ void inverse_xy(double2* lds, const Geom& g,
     for (int c = 0; c < kChunk; ++c) {
       const int e = base + c * kThreads + threadIdx.x;
       const int ky = e / nxh, kx = e - ky * nxh;
-      r[c] = e < tin ? spec[spec_index(i, ky, kx, z, ny, g.nt, g.nz)] : make_double2(0.0, 0.0);
+      r[c] = e < tin ? spec_ld(spec + spec_index(i, ky, kx, z, ny, g.nt, g.nz))
+                     : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int c = 0; c < kChunk; ++c) {
@@ -258,7 +259,7 @@ __device__ __forceinline__ void rhs_to_lds(double* pk, const Geom& g, const RhsA
 
 // A: per (z-plane, harmonic): BDF right-hand side from the history slices, forward
 // x and y passes
-template <int NX, int NY, int MAXE, int ORDER>
+template <int NX, int NY, int MAXE, int ORDER, class S>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1)))
 k_rhs_fft_xy(Geom g, RhsArgs a) {
   extern __shared__ double2 lds[];
@@ -267,17 +268,17 @@ k_rhs_fft_xy(Geom g, RhsArgs a) {
   const long long plane = (long long)z * g.zs;
   rhs_to_lds<NX, NY, ORDER, false>(reinterpret_cast<double*>(lds), g, a, i, plane, true, 1.0,
                                nullptr, false, true);
-  forward_xy<NX, NY, MAXE>(lds, g, a.spec, i, z);
+  forward_xy<NX, NY, MAXE>(lds, g, reinterpret_cast<S*>(a.spec), i, z);
 }
 
 // C: per (z-plane, harmonic): inverse y and x (C2R) passes, 1/N, store of q(s)
-template <int NX, int NY, int MAXE>
+template <int NX, int NY, int MAXE, class S>
 __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
   extern __shared__ double2 lds[];
   int z, i;
   plane_of_block(g.nz, z, i);
   const int nx = NX ? NX : g.nx, ny = NY ? NY : g.ny;
-  inverse_xy<NX, NY, MAXE>(lds, g, a.spec, i, z);
+  inverse_xy<NX, NY, MAXE>(lds, g, reinterpret_cast<const S*>(a.spec), i, z);
   double* out = a.out + i * g.hs + (long long)z * g.zs;
   const double* pk = reinterpret_cast<const double*>(lds);
   const double sc = a.scale;
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
 // it stays in LDS as the newest history plane of the next right-hand side (zero for
 // i >= M(s): a junction that grows the basis).  For i < M(next) the right-hand side
 // is formed in place and transformed forward into the same spectrum plane.
-template <int NX, int NY, int MAXE, int ORDER>
+template <int NX, int NY, int MAXE, int ORDER, class S>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1)))
 k_inv_rhs(Geom g, InvArgs c, RhsArgs a) {
   extern __shared__ double2 lds[];
@@ -301,10 +302,10 @@ k_inv_rhs(Geom g, InvArgs c, RhsArgs a) {
   const long long plane = (long long)z * g.zs;
   const bool inv = i < c.M;
   const bool rhs = i < a.M;
-  if (inv) inverse_xy<NX, NY, MAXE>(lds, g, c.spec, i, z);
+  if (inv) inverse_xy<NX, NY, MAXE>(lds, g, reinterpret_cast<const S*>(c.spec), i, z);
   rhs_to_lds<NX, NY, ORDER, true>(reinterpret_cast<double*>(lds), g, a, i, plane, inv, c.scale,
                               c.out + i * g.hs + plane, inv, rhs);
-  if (rhs) forward_xy<NX, NY, MAXE>(lds, g, a.spec, i, z);
+  if (rhs) forward_xy<NX, NY, MAXE>(lds, g, reinterpret_cast<S*>(a.spec), i, z);
 }
 
 namespace {
@@ -318,22 +319,22 @@ int pick_maxe(long long elems, int nthr) {
 }
 }  // namespace
 
-template <int NX, int NY, int MAXE>
+template <int NX, int NY, int MAXE, class S>
 static void* rhs_kernel(int order) {
   switch (order) {
-    case 1: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 1>;
-    case 2: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 2>;
-    case 3: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 3>;
+    case 1: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 1, S>;
+    case 2: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 2, S>;
+    case 3: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 3, S>;
   }
   return nullptr;
 }
 
-template <int NX, int NY, int MAXE>
+template <int NX, int NY, int MAXE, class S>
 static void* inv_rhs_kernel(int order) {
   switch (order) {
-    case 1: return (void*)k_inv_rhs<NX, NY, MAXE, 1>;
-    case 2: return (void*)k_inv_rhs<NX, NY, MAXE, 2>;
-    case 3: return (void*)k_inv_rhs<NX, NY, MAXE, 3>;
+    case 1: return (void*)k_inv_rhs<NX, NY, MAXE, 1, S>;
+    case 2: return (void*)k_inv_rhs<NX, NY, MAXE, 2, S>;
+    case 3: return (void*)k_inv_rhs<NX, NY, MAXE, 3, S>;
   }
   return nullptr;
 }
@@ -352,44 +353,46 @@ static size_t plane_lds_bytes(const Geom& g) {
 #include "wlc_plane128.h"
 namespace wlc {
 
-template <int ORDER>
+template <int ORDER, class S>
 static void* p64_kernel(int kind) {
-  return kind == 0 ? (void*)p64::k_rhs64<ORDER> : (void*)p64::k_inv_rhs64<ORDER>;
+  return kind == 0 ? (void*)p64::k_rhs64<ORDER, S> : (void*)p64::k_inv_rhs64<ORDER, S>;
 }
 
-template <int ORDER>
+template <int ORDER, class S>
 static void* p128_kernel(int kind) {
-  return kind == 0 ? (void*)p128::k_rhs128<ORDER> : (void*)p128::k_inv_rhs128<ORDER>;
+  return kind == 0 ? (void*)p128::k_rhs128<ORDER, S> : (void*)p128::k_inv_rhs128<ORDER, S>;
 }
 
-// kind 0: A (order), 1: C, 2: fused C + A (order of the next step); *threads = workgroup size
-static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds, int* threads) {
+// kind 0: A (order), 1: C, 2: fused C + A (order of the next step); *threads = workgroup
+// size; S = storage type of the half spectrum
+template <class S>
+static void* pick_plane_kernel_t(const Geom& g, int kind, int order, size_t* lds, int* threads) {
   *lds = plane_lds_bytes(g);
   *threads = kThreads;
   if (g.nx == 64 && g.ny == 64) {  // static 32 KB plane
     *lds = 0;
-    if (kind == 1) return (void*)p64::k_inv64;
+    if (kind == 1) return (void*)p64::k_inv64<S>;
     switch (order) {
-      case 1: return p64_kernel<1>(kind);
-      case 2: return p64_kernel<2>(kind);
-      case 3: return p64_kernel<3>(kind);
+      case 1: return p64_kernel<1, S>(kind);
+      case 2: return p64_kernel<2, S>(kind);
+      case 3: return p64_kernel<3, S>(kind);
     }
     return nullptr;
   }
   if (g.nx == 128 && g.ny == 128) {  // 128 KB plane in dynamic LDS, 512 threads
     *lds = sizeof(double2) * p128::kPlane;
     *threads = p128::kThreads;
-    if (kind == 1) return (void*)p128::k_inv128;
+    if (kind == 1) return (void*)p128::k_inv128<S>;
     switch (order) {
-      case 1: return p128_kernel<1>(kind);
-      case 2: return p128_kernel<2>(kind);
-      case 3: return p128_kernel<3>(kind);
+      case 1: return p128_kernel<1, S>(kind);
+      case 2: return p128_kernel<2, S>(kind);
+      case 3: return p128_kernel<3, S>(kind);
     }
     return nullptr;
   }
-#define WLC_PICK(NX, NY, MAXE)                                                     \
-  (kind == 0 ? rhs_kernel<NX, NY, MAXE>(order)                                     \
-             : kind == 1 ? (void*)k_ifft_xy<NX, NY, MAXE> : inv_rhs_kernel<NX, NY, MAXE>(order))
+#define WLC_PICK(NX, NY, MAXE)                                                       \
+  (kind == 0 ? rhs_kernel<NX, NY, MAXE, S>(order)                                    \
+             : kind == 1 ? (void*)k_ifft_xy<NX, NY, MAXE, S> : inv_rhs_kernel<NX, NY, MAXE, S>(order))
   if (specialised(g)) return WLC_PICK(32, 32, 0);
   switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
     case 8: return WLC_PICK(0, 0, 8);
@@ -399,6 +402,11 @@ static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds, 
   }
 #undef WLC_PICK
   return nullptr;
+}
+
+static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds, int* threads) {
+  return g.spec32 ? pick_plane_kernel_t<float2>(g, kind, order, lds, threads)
+                  : pick_plane_kernel_t<double2>(g, kind, order, lds, threads);
 }
 
 static hipError_t launch_plane(const Geom& g, int kind, int order, int planes, void** args,

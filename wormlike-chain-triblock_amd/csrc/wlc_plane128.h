@@ -59,10 +59,11 @@ __device__ __forceinline__ void pass2(double2 (&u)[16]) {
 }
 
 // Spectrum plane (i, z): workgroup-uniform base pointer and 32-bit element offsets
+template <class S>
 struct SpecPlane {
-  double2* p;
+  S* p;
   unsigned rs, ts;
-  __device__ __forceinline__ SpecPlane(double2* spec, const Geom& g, int i, int z)
+  __device__ __forceinline__ SpecPlane(S* spec, const Geom& g, int i, int z)
       : p(spec + spec_index(i, 0, 0, z, 128, g.nt, g.nz)),
         rs(unsigned(g.nt) * unsigned(g.nz) * kKxTile),
         ts(unsigned(g.nz) * kKxTile) {}
@@ -74,22 +75,23 @@ struct SpecPlane {
 // Inverse y pass of half-spectrum plane (i, z) into the y layout (unscaled).  Columns
 // kx = 0 and 64 enter as H0 + i H64 with H(ky) = (X(ky) + conj X(-ky)) / 2 (what a C2R
 // transform keeps of those two columns).
-__device__ __forceinline__ void y_inverse(double2* buf, const double2* __restrict__ spec,
+template <class S>
+__device__ __forceinline__ void y_inverse(double2* buf, const S* __restrict__ spec,
                                           const Geom& g, int i, int z) {
   const int t = threadIdx.x, c = t & 63, jy = t >> 6;
   const double2* tw = g.ay.tw;
   double2* hp = buf;  // packed column, one entry per ky (before the plane is written)
-  const SpecPlane sp(const_cast<double2*>(spec), g, i, z);
+  const SpecPlane<S> sp(const_cast<S*>(spec), g, i, z);
   double2 u[16];
   if (c != 0) {
-    const double2* col = sp.p + sp.at(jy, c);
+    const S* col = sp.p + sp.at(jy, c);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) u[r] = col[unsigned(8 * r) * sp.rs];
+    for (int r = 0; r < 16; ++r) u[r] = spec_ld(col + unsigned(8 * r) * sp.rs);
   }
   if (t < 128) {
     const int ky = t, kn = (128 - t) & 127;
-    const double2 a = sp.p[sp.at(ky, 0)], an = sp.p[sp.at(kn, 0)];
-    const double2 b = sp.p[sp.at(ky, 64)], bn = sp.p[sp.at(kn, 64)];
+    const double2 a = spec_ld(sp.p + sp.at(ky, 0)), an = spec_ld(sp.p + sp.at(kn, 0));
+    const double2 b = spec_ld(sp.p + sp.at(ky, 64)), bn = spec_ld(sp.p + sp.at(kn, 64));
     const double2 h0 = make_double2(0.5 * (a.x + an.x), 0.5 * (a.y - an.y));
     const double2 h1 = make_double2(0.5 * (b.x + bn.x), 0.5 * (b.y - bn.y));
     hp[ky] = make_double2(h0.x - h1.y, h0.y + h1.x);
@@ -159,7 +161,8 @@ __device__ __forceinline__ void x_inverse(double2* buf, const Geom& g) {
 
 // Forward x pass of the rows in the x layout, unpack into half-spectrum columns while
 // loading the y pass, forward y pass, store of spectrum plane (i, z).
-__device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ spec,
+template <class S>
+__device__ __forceinline__ void xy_forward(double2* buf, S* __restrict__ spec,
                                            const Geom& g, int i, int z) {
   const int b = threadIdx.x >> 3, j = threadIdx.x & 7;
   double2 u[16];
@@ -210,11 +213,11 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
   }
   pass2<-1>(u);
   // thread (c, jy) holds X(ky = jy + 8r) at u[p2(r)]
-  const SpecPlane sp(spec, g, i, z);
+  const SpecPlane<S> sp(spec, g, i, z);
   if (c != 0) {
-    double2* col = sp.p + sp.at(jy, c);
+    S* col = sp.p + sp.at(jy, c);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) col[unsigned(8 * r) * sp.rs] = u[p2(r)];
+    for (int r = 0; r < 16; ++r) spec_st(col + unsigned(8 * r) * sp.rs, u[p2(r)]);
   }
   __syncthreads();  // the plane buffer is free: exchange the packed column
   if (c == 0) {
@@ -227,8 +230,8 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
     // X64 = (P - conj P(-ky)) / 2i
     const int ky = threadIdx.x;
     const double2 p = buf[ky], pn = buf[(128 - ky) & 127];
-    sp.p[sp.at(ky, 0)] = make_double2(0.5 * (p.x + pn.x), 0.5 * (p.y - pn.y));
-    sp.p[sp.at(ky, 64)] = make_double2(0.5 * (p.y + pn.y), -0.5 * (p.x - pn.x));
+    spec_st(sp.p + sp.at(ky, 0), make_double2(0.5 * (p.x + pn.x), 0.5 * (p.y - pn.y)));
+    spec_st(sp.p + sp.at(ky, 64), make_double2(0.5 * (p.y + pn.y), -0.5 * (p.x - pn.x)));
   }
 }
 
@@ -292,21 +295,22 @@ __device__ __forceinline__ void rhs_rows(double2* buf, const RhsArgs& a, const G
 }
 
 // A: right-hand side from the history slices, forward x and y passes
-template <int ORDER>
+template <int ORDER, class S>
 __global__ void __launch_bounds__(kThreads) k_rhs128(Geom g, RhsArgs a) {
   extern __shared__ double2 buf[];
   int z, i;
   plane_of_block(g.nz, z, i);
   rhs_rows<ORDER, false>(buf, a, g, i, (long long)z * g.zs, true, 1.0, nullptr, false, true);
-  xy_forward(buf, a.spec, g, i, z);
+  xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z);
 }
 
 // C: inverse y and x passes, 1/N, store of q(s)
+template <class S>
 __global__ void __launch_bounds__(kThreads) k_inv128(Geom g, InvArgs c) {
   extern __shared__ double2 buf[];
   int z, i;
   plane_of_block(g.nz, z, i);
-  y_inverse(buf, c.spec, g, i, z);
+  y_inverse(buf, reinterpret_cast<const S*>(c.spec), g, i, z);
   x_inverse(buf, g);
   const long long plane = (long long)z * g.zs;
   RhsArgs none{};
@@ -315,7 +319,7 @@ __global__ void __launch_bounds__(kThreads) k_inv128(Geom g, InvArgs c) {
 }
 
 // C of step s fused with A of the next step
-template <int ORDER>
+template <int ORDER, class S>
 __global__ void __launch_bounds__(kThreads) k_inv_rhs128(Geom g, InvArgs c, RhsArgs a) {
   extern __shared__ double2 buf[];
   int z, i;
@@ -324,11 +328,11 @@ __global__ void __launch_bounds__(kThreads) k_inv_rhs128(Geom g, InvArgs c, RhsA
   const bool inv = i < c.M;
   const bool rhs = i < a.M;
   if (inv) {
-    y_inverse(buf, c.spec, g, i, z);
+    y_inverse(buf, reinterpret_cast<const S*>(c.spec), g, i, z);
     x_inverse(buf, g);
   }
   rhs_rows<ORDER, true>(buf, a, g, i, plane, inv, c.scale, c.out + i * g.hs + plane, inv, rhs);
-  if (rhs) xy_forward(buf, a.spec, g, i, z);
+  if (rhs) xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z);
 }
 
 }  // namespace p128

@@ -52,10 +52,11 @@ __device__ __forceinline__ void load_tw(Tw& w, const double2* __restrict__ tw64)
 // Spectrum plane (i, z) of the half spectrum: a workgroup-uniform base pointer and
 // 32-bit element offsets (ky stride 4 nt nz, kx-tile stride 4 nz), so the per-thread
 // address arithmetic stays in 32 bits.
+template <class S>
 struct SpecPlane {
-  double2* p;
+  S* p;
   unsigned rs, ts;
-  __device__ __forceinline__ SpecPlane(double2* spec, const Geom& g, int i, int z)
+  __device__ __forceinline__ SpecPlane(S* spec, const Geom& g, int i, int z)
       : p(spec + spec_index(i, 0, 0, z, 64, g.nt, g.nz)),
         rs(unsigned(g.nt) * unsigned(g.nz) * kKxTile),
         ts(unsigned(g.nz) * kKxTile) {}
@@ -68,21 +69,22 @@ struct SpecPlane {
 // kx = 0 and 32 enter as H0 + i H32 with H(ky) = (X(ky) + conj X(-ky)) / 2: the
 // inverse of H is the real part of the inverse of X, i.e. exactly what a C2R transform
 // keeps of those two columns.
-__device__ __forceinline__ void y_inverse(double2* buf, const double2* __restrict__ spec,
+template <class S>
+__device__ __forceinline__ void y_inverse(double2* buf, const S* __restrict__ spec,
                                           const Geom& g, int i, int z, const Tw& w) {
   const int t = threadIdx.x, c = t & 31, jy = t >> 5;
   double2* hp = buf;  // packed column, one entry per ky (before the plane is written)
-  const SpecPlane sp(const_cast<double2*>(spec), g, i, z);
+  const SpecPlane<S> sp(const_cast<S*>(spec), g, i, z);
   double2 v[8];
   if (c != 0) {
-    const double2* col = sp.p + sp.at(jy, c);
+    const S* col = sp.p + sp.at(jy, c);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = col[unsigned(8 * r) * sp.rs];
+    for (int r = 0; r < 8; ++r) v[r] = spec_ld(col + unsigned(8 * r) * sp.rs);
   }
   if (t < 64) {
     const int ky = t, kn = (64 - t) & 63;
-    const double2 a = sp.p[sp.at(ky, 0)], an = sp.p[sp.at(kn, 0)];
-    const double2 b = sp.p[sp.at(ky, 32)], bn = sp.p[sp.at(kn, 32)];
+    const double2 a = spec_ld(sp.p + sp.at(ky, 0)), an = spec_ld(sp.p + sp.at(kn, 0));
+    const double2 b = spec_ld(sp.p + sp.at(ky, 32)), bn = spec_ld(sp.p + sp.at(kn, 32));
     const double2 h0 = make_double2(0.5 * (a.x + an.x), 0.5 * (a.y - an.y));
     const double2 h1 = make_double2(0.5 * (b.x + bn.x), 0.5 * (b.y - bn.y));
     hp[ky] = make_double2(h0.x - h1.y, h0.y + h1.x);
@@ -145,7 +147,8 @@ __device__ __forceinline__ void x_inverse(double2* buf, const Tw& w) {
 
 // Forward x pass of the rows in the x layout, unpack into half-spectrum columns while
 // loading the y pass, forward y pass, store of spectrum plane (i, z).
-__device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ spec,
+template <class S>
+__device__ __forceinline__ void xy_forward(double2* buf, S* __restrict__ spec,
                                            const Geom& g, int i, int z, const Tw& w) {
   const int b = threadIdx.x >> 3, j = threadIdx.x & 7;
   double2 v[8];
@@ -190,11 +193,11 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
 #pragma unroll
   for (int r = 1; r < 8; ++r) v[r] = twiddle<-1>(v[r], w.y(r));
   Dft<8, -1>::run(v);
-  const SpecPlane sp(spec, g, i, z);
+  const SpecPlane<S> sp(spec, g, i, z);
   if (c != 0) {
-    double2* col = sp.p + sp.at(jy, c);
+    S* col = sp.p + sp.at(jy, c);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) col[unsigned(8 * r) * sp.rs] = v[r];
+    for (int r = 0; r < 8; ++r) spec_st(col + unsigned(8 * r) * sp.rs, v[r]);
   }
   __syncthreads();  // the plane buffer is free: exchange the packed column
   if (c == 0) {
@@ -207,8 +210,8 @@ __device__ __forceinline__ void xy_forward(double2* buf, double2* __restrict__ s
     // X32 = (P - conj P(-ky)) / 2i
     const int ky = threadIdx.x;
     const double2 p = buf[ky], pn = buf[(64 - ky) & 63];
-    sp.p[sp.at(ky, 0)] = make_double2(0.5 * (p.x + pn.x), 0.5 * (p.y - pn.y));
-    sp.p[sp.at(ky, 32)] = make_double2(0.5 * (p.y + pn.y), -0.5 * (p.x - pn.x));
+    spec_st(sp.p + sp.at(ky, 0), make_double2(0.5 * (p.x + pn.x), 0.5 * (p.y - pn.y)));
+    spec_st(sp.p + sp.at(ky, 32), make_double2(0.5 * (p.y + pn.y), -0.5 * (p.x - pn.x)));
   }
 }
 
@@ -284,7 +287,7 @@ __device__ __forceinline__ void rhs_rows(double2* buf, const RhsArgs& a, const G
 }
 
 // A: right-hand side from the history slices, forward x and y passes
-template <int ORDER>
+template <int ORDER, class S>
 __global__ void __launch_bounds__(256) k_rhs64(Geom g, RhsArgs a) {
   __shared__ double2 buf[kLds];
   int z, i;
@@ -292,17 +295,18 @@ __global__ void __launch_bounds__(256) k_rhs64(Geom g, RhsArgs a) {
   Tw w;
   load_tw(w, g.ax.tw);
   rhs_rows<ORDER, false>(buf, a, g, i, (long long)z * g.zs, true, 1.0, nullptr, false, true);
-  xy_forward(buf, a.spec, g, i, z, w);
+  xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z, w);
 }
 
 // C: inverse y and x passes, 1/N, store of q(s)
+template <class S>
 __global__ void __launch_bounds__(256) k_inv64(Geom g, InvArgs c) {
   __shared__ double2 buf[kLds];
   int z, i;
   plane_of_block(g.nz, z, i);
   Tw w;
   load_tw(w, g.ax.tw);
-  y_inverse(buf, c.spec, g, i, z, w);
+  y_inverse(buf, reinterpret_cast<const S*>(c.spec), g, i, z, w);
   x_inverse(buf, w);
   const long long plane = (long long)z * g.zs;
   RhsArgs none{};
@@ -311,7 +315,7 @@ __global__ void __launch_bounds__(256) k_inv64(Geom g, InvArgs c) {
 }
 
 // C of step s fused with A of the next step (see k_inv_rhs)
-template <int ORDER>
+template <int ORDER, class S>
 __global__ void __launch_bounds__(256) k_inv_rhs64(Geom g, InvArgs c, RhsArgs a) {
   __shared__ double2 buf[kLds];
   int z, i;
@@ -322,11 +326,11 @@ __global__ void __launch_bounds__(256) k_inv_rhs64(Geom g, InvArgs c, RhsArgs a)
   const bool inv = i < c.M;
   const bool rhs = i < a.M;
   if (inv) {
-    y_inverse(buf, c.spec, g, i, z, w);
+    y_inverse(buf, reinterpret_cast<const S*>(c.spec), g, i, z, w);
     x_inverse(buf, w);
   }
   rhs_rows<ORDER, true>(buf, a, g, i, plane, inv, c.scale, c.out + i * g.hs + plane, inv, rhs);
-  if (rhs) xy_forward(buf, a.spec, g, i, z, w);
+  if (rhs) xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z, w);
 }
 
 }  // namespace p64

@@ -48,6 +48,7 @@ __device__ __forceinline__ void z_transform(double2 (&v)[kM], double2* lds, cons
 
 // generic kernel B: any nz <= 256; TX columns of one ky row per workgroup, z transforms
 // through LDS eight harmonics at a time (block-wide Stockham passes)
+template <class S>
 __global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
   extern __shared__ double2 lds[];
   const int nz = g.nz, ny = g.ny, nxh = g.nxh;
@@ -57,10 +58,10 @@ __global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
   const int kx = blockIdx.x * TX + tx, ky = blockIdx.y;
   const bool valid = kx < nxh;
   const long long istride = spec_index(1, 0, 0, 0, ny, g.nt, nz);
-  double2* p = a.spec + spec_index(0, ky, valid ? kx : 0, z, ny, g.nt, nz);
+  S* p = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, valid ? kx : 0, z, ny, g.nt, nz);
   double2 v[kM];
 #pragma unroll
-  for (int i = 0; i < kM; ++i) v[i] = valid ? p[i * istride] : make_double2(0.0, 0.0);
+  for (int i = 0; i < kM; ++i) v[i] = valid ? spec_ld(p + i * istride) : make_double2(0.0, 0.0);
   const int lidx = tx * nz + z;
   z_transform<-1>(v, lds, g.az, TX, lidx);
   if (a.solve && valid) {
@@ -72,7 +73,7 @@ __global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
   z_transform<+1>(v, lds, g.az, TX, lidx);
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < kM; ++i) p[i * istride] = v[i];
+    for (int i = 0; i < kM; ++i) spec_st(p + i * istride, v[i]);
   }
 }
 
@@ -97,11 +98,12 @@ __device__ __forceinline__ void wave_sync() {
 // the tiles with coalesced 1 KB stores.
 constexpr int kRegion = kHB * kPadBatch + 2;  // per-wave LDS region (padded stride)
 
-__device__ __forceinline__ void tile_load(double2 (&r)[kHB], const double2* __restrict__ tile,
+template <class S>
+__device__ __forceinline__ void tile_load(double2 (&r)[kHB], const S* __restrict__ tile,
                                           long long istride, int b0, int t) {
 #pragma unroll
   for (int h = 0; h < kHB; ++h)
-    r[h] = (b0 + h < kM) ? tile[(long long)(b0 + h) * istride + t] : make_double2(0.0, 0.0);
+    r[h] = (b0 + h < kM) ? spec_ld(tile + (long long)(b0 + h) * istride + t) : make_double2(0.0, 0.0);
 }
 
 __device__ __forceinline__ void tile_to_lds(const double2 (&r)[kHB], double2* lds, int t) {
@@ -110,8 +112,9 @@ __device__ __forceinline__ void tile_to_lds(const double2 (&r)[kHB], double2* ld
   for (int h = 0; h < kHB; ++h) lds[w * kRegion + h * kPadBatch + pad8(z)] = r[h];
 }
 
+template <class S>
 __device__ __forceinline__ void z64_forward(double2 (&v)[kM], double2* lds, int wv, int lane,
-                                            const double2 (&tw)[8], const double2* __restrict__ tile,
+                                            const double2 (&tw)[8], const S* __restrict__ tile,
                                             long long istride) {
   const int h = lane >> 3, j = lane & 7, t = threadIdx.x;
   double2* wl = lds + wv * kRegion;
@@ -146,8 +149,9 @@ __device__ __forceinline__ void z64_forward(double2 (&v)[kM], double2* lds, int 
   }
 }
 
+template <class S>
 __device__ __forceinline__ void z64_inverse(const double2 (&v)[kM], double2* lds, int wv, int lane,
-                                            const double2 (&tw)[8], double2* __restrict__ tile,
+                                            const double2 (&tw)[8], S* __restrict__ tile,
                                             long long istride) {
   const int h = lane >> 3, j = lane & 7, t = threadIdx.x;
   const int tz = t >> 2, tw4 = t & 3;
@@ -179,11 +183,11 @@ __device__ __forceinline__ void z64_inverse(const double2 (&v)[kM], double2* lds
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (b0 + q < kM)
-        tile[(long long)(b0 + q) * istride + t] = lds[tw4 * kRegion + q * kPadBatch + pad8(tz)];
+        spec_st(tile + (long long)(b0 + q) * istride + t, lds[tw4 * kRegion + q * kPadBatch + pad8(tz)]);
   }
 }
 
-template <int OCC>
+template <int OCC, class S>
 __global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
   __shared__ double2 lds[4 * kRegion];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -193,7 +197,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
   const bool valid = kx < nxh;
   const long long istride = spec_index(1, 0, 0, 0, ny, g.nt, 64);
   // the whole 4-kx tile is allocated even where kx >= nxh (the layout pads to 4)
-  double2* tile = a.spec + spec_index(0, ky, t4 * kKxTile, 0, ny, g.nt, 64);
+  S* tile = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 0, ny, g.nt, 64);
   double2 tw[8];
   {
     const int j = lane & 7;
@@ -296,7 +300,7 @@ __device__ __forceinline__ void z64_inv_regs(double2 (&v)[N], double2* wl, int l
   }
 }
 
-template <int OCC>
+template <int OCC, class S>
 __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
   __shared__ double2 lds[4 * kRegionS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -324,7 +328,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
   const long long istride = spec_index(1, 0, 0, 0, ny, nt, 64);
   // column kx of the tile: element (i, z) at i * istride + 4 z (the tile is allocated
   // even where kx >= nxh)
-  double2* col = a.spec + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, 64) + kxl;
+  S* col = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, 64) + kxl;
   double2* wl = lds + wv * kRegionS;
   double2* xch = lds + (wv & ~1) * kRegionS;  // the Lo wave's region of this column
   double2 tw[8];
@@ -339,7 +343,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
   // phase 1: load, forward z transform, rotate into the k frame; Lo: forward elimination
   if (part == 0 && real_col) {
 #pragma unroll
-    for (int h = 0; h < PartLo::NH; ++h) v[h] = col[(long long)(PartLo::B0 + h) * istride + 4 * lane];
+    for (int h = 0; h < PartLo::NH; ++h) v[h] = spec_ld(col + (long long)(PartLo::B0 + h) * istride + 4 * lane);
     z64_fwd_regs<PartLo::NH>(v, wl, lane, tw);
     if (active) {
       PartLo::to_frame(v, f);
@@ -347,7 +351,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
     }
   } else if (real_col) {
 #pragma unroll
-    for (int h = 0; h < PartHi::NH; ++h) v[h] = col[(long long)(PartHi::B0 + h) * istride + 4 * lane];
+    for (int h = 0; h < PartHi::NH; ++h) v[h] = spec_ld(col + (long long)(PartHi::B0 + h) * istride + 4 * lane);
     z64_fwd_regs<PartHi::NH>(v, wl, lane, tw);
     if (active) PartHi::to_frame(v, f);
   }
@@ -363,12 +367,12 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
     }
     z64_inv_regs<PartLo::NH>(v, wl, lane, tw);
 #pragma unroll
-    for (int h = 0; h < PartLo::NH; ++h) col[(long long)(PartLo::B0 + h) * istride + 4 * lane] = v[h];
+    for (int h = 0; h < PartLo::NH; ++h) spec_st(col + (long long)(PartLo::B0 + h) * istride + 4 * lane, v[h]);
   } else if (real_col) {
     if (active) PartHi::from_frame(v, f);
     z64_inv_regs<PartHi::NH>(v, wl, lane, tw);
 #pragma unroll
-    for (int h = 0; h < PartHi::NH; ++h) col[(long long)(PartHi::B0 + h) * istride + 4 * lane] = v[h];
+    for (int h = 0; h < PartHi::NH; ++h) spec_st(col + (long long)(PartHi::B0 + h) * istride + 4 * lane, v[h]);
   }
 }
 
@@ -422,7 +426,7 @@ __device__ __forceinline__ void z128_exchange(double2 (&v)[N], double2* own, con
   }
 }
 
-template <int OCC>
+template <int OCC, class S>
 __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   __shared__ double2 lds[4 * kRegion128];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -446,7 +450,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   const bool real_col = kx < nxh;
   const long long istride = spec_index(1, 0, 0, 0, ny, nt, 128);
   // column kx: element (i, z) at i * istride + 4 z; this wave's half starts at z = 64 p
-  double2* col = a.spec + spec_index(0, ky, t4 * kKxTile, 64 * p, ny, nt, 128) + kxl;
+  S* col = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 64 * p, ny, nt, 128) + kxl;
   double2* wl = lds + wv * kRegion128;
   const double2* partner = lds + (wv ^ 1) * kRegion128;
   double2 tw[8];
@@ -459,7 +463,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   double2 v[kM];
   if (real_col) {
 #pragma unroll
-    for (int h = 0; h < kM; ++h) v[h] = col[(long long)h * istride + 4 * lane];
+    for (int h = 0; h < kM; ++h) v[h] = spec_ld(col + (long long)h * istride + 4 * lane);
   }
   z128_exchange<-1, kM>(v, wl, partner, lane, p, t, real_col);
   __syncthreads();  // the exchange regions are reused by the 64-point transforms
@@ -480,7 +484,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   z128_exchange<+1, kM>(v, wl, partner, lane, p, t, real_col);
   if (real_col) {
 #pragma unroll
-    for (int h = 0; h < kM; ++h) col[(long long)h * istride + 4 * lane] = v[h];
+    for (int h = 0; h < kM; ++h) spec_st(col + (long long)h * istride + 4 * lane, v[h]);
   }
 }
 
@@ -489,7 +493,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
 // wv with wv ^ 1, the crossing tridiagonal chains pair wv with wv ^ 2 through the region of
 // the part-0 wave of the same z half.  Each wave holds about half the harmonics, so two
 // waves fit per SIMD at Lmax 7 and 8.
-template <int OCC>
+template <int OCC, class S>
 __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
   __shared__ double2 lds[4 * kRegion128];
   static_assert(kXch + 2 <= kRegion128, "crossing chains fit a wave region");
@@ -512,7 +516,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
   const int kx = t4 * kKxTile + kxl;
   if (kx >= nxh) return;  // a padding column (uniform over the workgroup)
   const long long istride = spec_index(1, 0, 0, 0, ny, nt, 128);
-  double2* col = a.spec + spec_index(0, ky, t4 * kKxTile, 64 * p, ny, nt, 128) + kxl;
+  S* col = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 64 * p, ny, nt, 128) + kxl;
   double2* wl = lds + wv * kRegion128;
   const double2* partner = lds + (wv ^ 1) * kRegion128;
   double2* xch = lds + p * kRegion128;  // region of the part-0 wave of this z half
@@ -527,11 +531,11 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
     const double2 t = g.az.tw[lane];  // exp(-2 pi i m / 128)
     if (part == 0) {
 #pragma unroll
-      for (int h = 0; h < PartLo::NH; ++h) v[h] = col[(long long)(PartLo::B0 + h) * istride + 4 * lane];
+      for (int h = 0; h < PartLo::NH; ++h) v[h] = spec_ld(col + (long long)(PartLo::B0 + h) * istride + 4 * lane);
       z128_exchange<-1, PartLo::NH, kNV>(v, wl, partner, lane, p, t, true);
     } else {
 #pragma unroll
-      for (int h = 0; h < PartHi::NH; ++h) v[h] = col[(long long)(PartHi::B0 + h) * istride + 4 * lane];
+      for (int h = 0; h < PartHi::NH; ++h) v[h] = spec_ld(col + (long long)(PartHi::B0 + h) * istride + 4 * lane);
       z128_exchange<-1, PartHi::NH, kNV>(v, wl, partner, lane, p, t, true);
     }
   }
@@ -581,7 +585,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
     }
     z128_exchange<+1, PartLo::NH, kNV>(v, wl, partner, lane, p, t, true);
 #pragma unroll
-    for (int h = 0; h < PartLo::NH; ++h) col[(long long)(PartLo::B0 + h) * istride + 4 * lane] = v[h];
+    for (int h = 0; h < PartLo::NH; ++h) spec_st(col + (long long)(PartLo::B0 + h) * istride + 4 * lane, v[h]);
   } else {
     z64_inv_regs<PartHi::NH>(v, wl, lane, tw);
     if (p == 1) {
@@ -590,7 +594,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
     }
     z128_exchange<+1, PartHi::NH, kNV>(v, wl, partner, lane, p, t, true);
 #pragma unroll
-    for (int h = 0; h < PartHi::NH; ++h) col[(long long)(PartHi::B0 + h) * istride + 4 * lane] = v[h];
+    for (int h = 0; h < PartHi::NH; ++h) spec_st(col + (long long)(PartHi::B0 + h) * istride + 4 * lane, v[h]);
   }
 }
 
@@ -598,40 +602,47 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
 
 using namespace WLC_CAT(per_l_, WLC_L);
 
-hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hipStream_t st) {
+namespace {
+template <class S>
+hipError_t zsolve_launch(const Geom& g, const SolveArgs& a, hipStream_t st) {
   if (a.M != kM) return hipErrorInvalidValue;
   if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
     // two workgroups per 4-kx tile
     const int blocks = 2 * g.nt * g.ny;
-    if (a.occ == 3) hipLaunchKernelGGL(k_zsolve64s<3>, dim3(blocks), dim3(256), 0, st, g, a);
-    else hipLaunchKernelGGL(k_zsolve64s<2>, dim3(blocks), dim3(256), 0, st, g, a);
+    if (a.occ == 3) hipLaunchKernelGGL((k_zsolve64s<3, S>), dim3(blocks), dim3(256), 0, st, g, a);
+    else hipLaunchKernelGGL((k_zsolve64s<2, S>), dim3(blocks), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (g.nz == 128 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
     // one workgroup (one column, harmonics split over two waves per z half) per column
     const int blocks = 4 * g.nt * g.ny;
-    hipLaunchKernelGGL(k_zsolve128s<2>, dim3(blocks), dim3(256), 0, st, g, a);
+    hipLaunchKernelGGL((k_zsolve128s<2, S>), dim3(blocks), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (g.nz == 128 && !a.generic) {
     // two workgroups (two columns each) per 4-kx tile
     const int blocks = 2 * g.nt * g.ny;
-    hipLaunchKernelGGL(k_zsolve128<(kL >= 7 ? 1 : 2)>, dim3(blocks), dim3(256), 0, st, g, a);
+    hipLaunchKernelGGL((k_zsolve128<(kL >= 7 ? 1 : 2), S>), dim3(blocks), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (g.nz == 64 && !a.generic) {
     static_assert(kKxTile == 4, "one wave per kx of a tile");
-    if (a.occ == 2) hipLaunchKernelGGL(k_zsolve64<2>, dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
-    else hipLaunchKernelGGL(k_zsolve64<1>, dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
+    if (a.occ == 2) hipLaunchKernelGGL((k_zsolve64<2, S>), dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
+    else hipLaunchKernelGGL((k_zsolve64<1, S>), dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (a.TX * g.nz > 256 || a.TX < 1) return hipErrorInvalidValue;
   const size_t lds = sizeof(double2) * (size_t)kHB * a.TX * g.nz;
-  hipError_t e = hipFuncSetAttribute((const void*)k_zsolve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = hipFuncSetAttribute((const void*)k_zsolve<S>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid((g.nxh + a.TX - 1) / a.TX, g.ny), block(a.TX * g.nz);
-  hipLaunchKernelGGL(k_zsolve, grid, block, lds, st, g, a);
+  hipLaunchKernelGGL(k_zsolve<S>, grid, block, lds, st, g, a);
   return hipGetLastError();
+}
+}  // namespace
+
+hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hipStream_t st) {
+  return g.spec32 ? zsolve_launch<float2>(g, a, st) : zsolve_launch<double2>(g, a, st);
 }
 
 namespace WLC_CAT(per_l_, WLC_L) {

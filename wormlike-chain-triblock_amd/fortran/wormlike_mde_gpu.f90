@@ -22,7 +22,8 @@ module wlc_mde_c
      real(c_double) :: lambda(3)
      integer(c_int) :: keep_qstar
      integer(c_int) :: ckpt_every            ! 0: automatic forward store / checkpoints
-     integer(c_int) :: reserved(6)
+     integer(c_int) :: spectrum_fp32
+     integer(c_int) :: reserved(5)
   end type wlc_params
 
   interface
@@ -144,6 +145,7 @@ contains
     end do
     p%keep_qstar = 0
     p%ckpt_every = 0
+    p%spectrum_fp32 = 0
     p%reserved = 0
     call wlc_check(wlc_mde_create(p, handle), 'wlc_mde_create')
     if (.not. allocated(w_blk)) allocate(w_blk(0:SIDEx*SIDEy*SIDEz-1, n_blk))

@@ -39,7 +39,8 @@ class WlcParams(ctypes.Structure):
         ("Nblk", ctypes.c_int * 3), ("Lmax", ctypes.c_int * 3),
         ("lam", ctypes.c_double * 3),
         ("keep_qstar", ctypes.c_int), ("ckpt_every", ctypes.c_int),
-        ("reserved", ctypes.c_int * 6),
+        ("spectrum_fp32", ctypes.c_int),
+        ("reserved", ctypes.c_int * 5),
     ]
 
 
@@ -113,7 +114,8 @@ def block_lengths(Ns, fractions):
 class WormlikeMDE:
     """One chain architecture on one grid; all state lives on the GPU."""
 
-    def __init__(self, shape, box, Nblk, Lmax, lam, keep_qstar=False, ckpt_every=0):
+    def __init__(self, shape, box, Nblk, Lmax, lam, keep_qstar=False, ckpt_every=0,
+                 spectrum_fp32=False):
         load()
         nz, ny, nx = shape
         p = WlcParams()
@@ -129,6 +131,7 @@ class WormlikeMDE:
             p.lam[b] = float(lam[b])
         p.keep_qstar = 1 if keep_qstar else 0
         p.ckpt_every = int(ckpt_every)
+        p.spectrum_fp32 = 1 if spectrum_fp32 else 0
         self.params = p
         self.shape = (nz, ny, nx)
         self.N = nx * ny * nz
