@@ -113,6 +113,12 @@ int wlc_mde_sync(wlc_handle* h);
 /* the HIP stream (hipStream_t) all work of the handle is queued on */
 void* wlc_mde_stream(wlc_handle* h);
 
+/* Switch the storage precision of the half spectrum (see wlc_params.spectrum_fp32) between
+ * sweeps, e.g. FP32 while an SCFT iteration is far from convergence and FP64 for the final
+ * iterations and the free energy.  Waits for queued work; the next wlc_mde_density needs
+ * new sweeps.  No counterpart in the reference. */
+int wlc_mde_set_spectrum_fp32(wlc_handle* h, int on);
+
 /* Per-kernel timing with HIP events.  enable: 0 off; 1 every launch of the step kernels
  * is bracketed by events on the stream it runs on, read back when the host waits anyway
  * (wlc_mde_density*, wlc_mde_get_timing, wlc_mde_set_timing), so the sweeps keep running

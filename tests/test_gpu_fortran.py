@@ -191,3 +191,23 @@ def test_split_operator_call_slot_dispatches_to_the_same_solve(wlc, tmp_path):
     got, wgot, _ = run_driver(b, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05,
                               flags=["--split-operator-slot"])
     assert np.array_equal(got, ref) and np.array_equal(wgot, wref)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran binding not built (no flang)")
+def test_fortran_fp32_spectrum_then_fp64(wlc, tmp_path):
+    """--spectrum-fp32 <tol>: iterations with the FP32 half spectrum until the residual is
+    below tol, then FP64.  With a tolerance every residual meets, only the first iteration
+    runs in FP32: its F and ln Q differ from the FP64 run by rounding-level amounts
+    (stated: 1e-6 kT, 1e-6), and the trajectory stays that close afterwards."""
+    rng = np.random.default_rng(9)
+    w = 0.5 * rng.standard_normal((3,) + CASE["shape"])
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    ref, wref, _ = run_driver(a, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05)
+    got, wgot, _ = run_driver(b, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05,
+                              flags=["--spectrum-fp32", "1e300"])
+    assert not np.array_equal(got[0, 1:3], ref[0, 1:3])    # iteration 1 really ran in FP32
+    assert np.abs(got[:, 1] - ref[:, 1]).max() < 1e-6       # F, kT
+    assert np.abs(got[:, 2] - ref[:, 2]).max() < 1e-6       # ln Q
+    assert np.abs(wgot - wref).max() < 1e-6 * np.abs(wref).max()

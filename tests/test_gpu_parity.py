@@ -334,3 +334,34 @@ def test_fp32_spectrum_matches_oracle_within_stated_tolerance(wlc, oracle, case)
     assert _rel(q32, orc.slice(q, sum(Nblk))) < FP32_TOL
     # the mode really stores FP32: the result differs from the FP64 path
     assert _rel(p32, p64) > 1e-12
+
+
+def test_spectrum_precision_switch_between_sweeps(wlc):
+    """wlc_mde_set_spectrum_fp32 switches the storage precision between sweeps: the results
+    equal, bit for bit, those of a handle created in that mode, and sweeps from before the
+    switch are not reused."""
+    shape, box, Nblk, L, lam = (8, 16, 16), (2.0, 3.0, 3.0), (4, 5, 3), (2, 3, 1), (1.0, 0.6, 1.5)
+    rng = np.random.default_rng(43)
+    w = rng.standard_normal((3,) + shape)
+
+    def run(g):
+        g.set_w(w)
+        g.MDE_q()
+        g.MDE_qstar()
+        return g.density()
+
+    ref = {}
+    for fp32 in (False, True):
+        g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, spectrum_fp32=fp32)
+        ref[fp32] = run(g)
+        g.close()
+    g = wlc.WormlikeMDE(shape, box, Nblk, L, lam)
+    run(g)
+    g.set_spectrum_fp32(True)
+    with pytest.raises(wlc.WlcError):
+        g.density()
+    for fp32 in (True, False, True):
+        g.set_spectrum_fp32(fp32)
+        phi, Q, td = run(g)
+        assert np.array_equal(phi, ref[fp32][0]) and Q == ref[fp32][1] and td == ref[fp32][2]
+    g.close()

@@ -739,8 +739,8 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     return fail(WLC_EHIP, "event creation failed");
   }
   const size_t store_bytes = sizeof(double) * (size_t)h->off[h->Ns + 1];
-  const size_t spec_elem = p->spectrum_fp32 ? sizeof(float2) : sizeof(double2);
-  const size_t spec_bytes = spec_elem * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz;
+  // sized for FP64 elements in either mode, so that wlc_mde_set_spectrum_fp32 can switch
+  const size_t spec_bytes = sizeof(double2) * (size_t)h->Mmax * p->ny * wlc::kx_tiles(h->nxh) * wlc::kKxTile * p->nz;
   const size_t slot_bytes = sizeof(double) * (size_t)slot_size(h);
   const size_t field_bytes = sizeof(double) * (size_t)p->nblk * h->hs;
   {
@@ -1013,6 +1013,17 @@ int wlc_mde_sync(wlc_handle* h) {
 }
 
 void* wlc_mde_stream(wlc_handle* h) { return h ? (void*)h->st : nullptr; }
+
+int wlc_mde_set_spectrum_fp32(wlc_handle* h, int on) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  if (on != 0 && on != 1) return fail(WLC_EINVAL, "spectrum_fp32 must be 0 or 1");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->st));  // no queued sweep still uses the old element type
+  h->p.spectrum_fp32 = on;
+  h->geom.spec32 = on;
+  h->fwd_done = h->bwd_done = false;  // earlier sweeps are not reused across the switch
+  return WLC_OK;
+}
 
 int wlc_mde_set_timing(wlc_handle* h, int enable) {
   if (!h) return fail(WLC_EINVAL, "NULL handle");

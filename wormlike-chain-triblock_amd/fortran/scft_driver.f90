@@ -8,6 +8,10 @@
 !            fields of input.bin (the reference's wAB_init_type < 0 -> readW,
 !            SCFT_control.f90:79-83)
 !   --dumpW  write the final fields to W0.dat (W_dump, SCFT_dump.f90:273-293)
+!   --spectrum-fp32 <tol>  iterate with the half spectrum stored as FP32 (all arithmetic
+!            FP64) until the largest residual is below tol, then switch to FP64 for the
+!            remaining iterations; convergence is only declared in FP64, so the final
+!            fields, residual and free energy come from FP64 sweeps
 !   --split-operator-slot  set up and solve through the split-operator call slot
 !            (split_operator_MDE_init / AB_diblock_sph_driver) instead of MDE_setup /
 !            MDE_q / MDE_qstar (same results: the slot runs the same solve)
@@ -28,7 +32,8 @@ program scft_driver
   implicit none
   character(len=1024) :: fin, fout, fw, arg
   integer :: u, hdr(16), max_iter, method, it, a, b, n, nargs, done, i, npos, stat
-  logical :: read_w, dump_w, sph_slot
+  logical :: read_w, dump_w, sph_slot, fp32_on, next_is_tol
+  real(8) :: fp32_tol
   integer(8) :: c0, c1, rate
   real(8) :: box(3), lam(3), chi9(9), chi(3, 3), f(3), tol, mix(2)
   logical :: converged
@@ -36,10 +41,17 @@ program scft_driver
   nargs = command_argument_count()
   fin = ''; fout = ''; fw = ''
   read_w = .false.; dump_w = .false.; sph_slot = .false.
+  fp32_on = .false.; next_is_tol = .false.; fp32_tol = 0d0
   npos = 0
   do i = 1, nargs
      call get_command_argument(i, arg)
-     if (trim(arg) == '--readW') then
+     if (next_is_tol) then
+        read(arg, *) fp32_tol
+        fp32_on = fp32_tol > 0d0
+        next_is_tol = .false.
+     else if (trim(arg) == '--spectrum-fp32') then
+        next_is_tol = .true.
+     else if (trim(arg) == '--readW') then
         read_w = .true.
      else if (trim(arg) == '--dumpW') then
         dump_w = .true.
@@ -52,9 +64,9 @@ program scft_driver
         if (npos == 3) fw = arg
      end if
   end do
-  if (npos < 2 .or. npos > 3) then
+  if (npos < 2 .or. npos > 3 .or. next_is_tol) then
      write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin] [--readW] [--dumpW] &
-          &[--split-operator-slot]'
+          &[--spectrum-fp32 <tol>] [--split-operator-slot]'
      error stop 2
   end if
 
@@ -91,6 +103,7 @@ program scft_driver
      end if
   end if
   call scft_iterate_init()
+  if (fp32_on) call MDE_set_spectrum_fp32(.true.)
   f = 0d0
   do a = 1, n_blk
      f(a) = dble(Ns_blk(a)) / dble(Nmax)
@@ -110,6 +123,12 @@ program scft_driver
      call density()
      call free_energy(chi)
      call scft_update(it, chi, f, tol, converged)
+     if (fp32_on .and. (converged .or. maxval(resid(1:n_blk)) < fp32_tol)) then
+        ! the rest, including the convergence test, in FP64
+        call MDE_set_spectrum_fp32(.false.)
+        fp32_on = .false.
+        converged = .false.
+     end if
      write(u, '(i6, 3es25.16e3, 3es25.16e3, 3es25.16e3, es25.16e3)') it, FE_global, pff_global, &
           maxval(resid(1:n_blk)), resid, (sum(R_blk(:, a)) / n, a = 1, n_blk), &
           (0d0, a = n_blk + 1, 3), maxval(abs(w_blk))

@@ -62,6 +62,12 @@ module wlc_mde_c
        real(c_double) :: Q, totden
      end function wlc_mde_density
 
+     integer(c_int) function wlc_mde_set_spectrum_fp32(h, on) bind(C, name="wlc_mde_set_spectrum_fp32")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h
+       integer(c_int), value :: on
+     end function wlc_mde_set_spectrum_fp32
+
      subroutine wlc_mde_destroy(h) bind(C, name="wlc_mde_destroy")
        import :: c_ptr
        type(c_ptr), value :: h
@@ -171,6 +177,16 @@ contains
     logical, optional :: R2C_FFT
     call wlc_check(wlc_mde_backward(handle), 'wlc_mde_backward')
   end subroutine MDE_qstar
+
+  ! FP32 storage of the half spectrum (all arithmetic stays FP64) on or off for the next
+  ! sweeps: an SCFT loop can iterate in FP32 far from convergence and finish in FP64
+  subroutine MDE_set_spectrum_fp32(on)
+    logical, intent(in) :: on
+    integer(c_int) :: v
+    v = 0
+    if (on) v = 1
+    call wlc_check(wlc_mde_set_spectrum_fp32(handle, v), 'wlc_mde_set_spectrum_fp32')
+  end subroutine MDE_set_spectrum_fp32
 
   ! Call slot of the reference's split-operator MDE (split_operator_MDE_mod.f90:24-115
   ! split_operator_MDE_init, called whenever the cell changes; :120-238

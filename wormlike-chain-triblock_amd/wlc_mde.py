@@ -50,6 +50,7 @@ EXPORTS = [
     "wlc_mde_get_q", "wlc_mde_slice_M", "wlc_mde_block_of", "wlc_mde_sync",
     "wlc_mde_stream", "wlc_mde_set_timing", "wlc_mde_get_timing", "wlc_mde_event_record",
     "wlc_mde_event_elapsed", "wlc_mde_destroy", "wlc_mde_last_error", "wlc_test_solve",
+    "wlc_mde_set_spectrum_fp32",
 ]
 
 _lib = None
@@ -81,6 +82,7 @@ def load(path=LIB_PATH):
     L.wlc_mde_stream.argtypes = [vp]
     L.wlc_mde_stream.restype = vp
     L.wlc_mde_set_timing.argtypes = [vp, ctypes.c_int]
+    L.wlc_mde_set_spectrum_fp32.argtypes = [vp, ctypes.c_int]
     L.wlc_mde_get_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
     L.wlc_mde_event_record.argtypes = [vp, ctypes.c_int]
     L.wlc_mde_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
@@ -205,6 +207,9 @@ class WormlikeMDE:
         _check(_lib.wlc_mde_sync(self.h))
 
     # --- measurement ---------------------------------------------------------------
+    def set_spectrum_fp32(self, on):
+        _check(_lib.wlc_mde_set_spectrum_fp32(self.h, 1 if on else 0))
+
     def set_timing(self, enable):
         """0/False off, 1/True live (sweeps stay concurrent), 2 sweeps serialized."""
         _check(_lib.wlc_mde_set_timing(self.h, int(enable)))
