@@ -113,43 +113,55 @@ def workload_w128(n=128, seed=2):
 def bench_grid128(W, args):
     """FP64 forward + backward sweeps of the 128^3 Lmax=8 workload (349 GB of forward store
     per sweep would not fit: checkpoints every 16 contour steps, segments recomputed during
-    the backward sweep)."""
+    the backward sweep).  With --spectrum-fp32 the same sweeps also run with the half
+    spectrum stored as FP32, and the line reports that rate with its phi / Q / F (chiN = 20)
+    differences from the FP64 run."""
     n, Ns, L, lam, box, ck = 128, 256, 8, 0.5, 8.0, 16
     Nblk = W.block_lengths(Ns, [1 / 3, 1 / 3, 1 / 3])
     M, N, Nk = (L + 1) ** 2, n ** 3, n * n * (n // 2 + 1)
-    g = W.WormlikeMDE((n,) * 3, (box,) * 3, Nblk, (L,) * 3, (lam,) * 3, ckpt_every=ck)
-    g.set_w(workload_w128(n))
+    w = workload_w128(n)
+    chi = np.full((3, 3), 20.0) - 20.0 * np.eye(3)
     steps = max(1, min(args.steps, 3))
-    for _ in range(min(args.warmup, 1)):
+    runs = {}
+    for fp32 in ((False, True) if args.spectrum_fp32 else (False,)):
+        g = W.WormlikeMDE((n,) * 3, (box,) * 3, Nblk, (L,) * 3, (lam,) * 3, ckpt_every=ck,
+                          spectrum_fp32=fp32)
+        g.set_w(w)
+        for _ in range(min(args.warmup, 1)):
+            g.MDE_q()
+            g.MDE_qstar()
+            g.density_device()
+        g.sync()
+        g.event_record(0)
+        Q = td = None
+        for _ in range(steps):
+            g.MDE_q()
+            g.MDE_qstar()
+            Q, td = g.density_device()
+        g.event_record(1)
+        ms = g.event_elapsed(0, 1)
+        g.sync()
+        if not (np.isfinite(Q) and np.isfinite(td)):
+            raise RuntimeError("non-finite Q/totden")
+        g.set_timing(2)
         g.MDE_q()
         g.MDE_qstar()
-        g.density_device()
-    g.sync()
-    g.event_record(0)
-    Q = td = None
-    for _ in range(steps):
-        g.MDE_q()
-        g.MDE_qstar()
-        Q, td = g.density_device()
-    g.event_record(1)
-    ms = g.event_elapsed(0, 1)
-    g.sync()
-    if not (np.isfinite(Q) and np.isfinite(td)):
-        raise RuntimeError("non-finite Q/totden")
-    g.set_timing(2)
-    g.MDE_q()
-    g.MDE_qstar()
-    g.density_device()
-    kms, kcnt = g.get_timing()
-    g.set_timing(0)
-    g.close()
+        phi, Q, _ = g.density()
+        kms, kcnt = g.get_timing()
+        g.set_timing(0)
+        g.close()
+        runs[fp32] = (ms, kms, kcnt, phi, Q, free_energy(w, phi, Q, chi, box))
+    fp32 = args.spectrum_fp32
+    ms, kms, kcnt, phi, Q, F = runs[fp32]
     contour_steps = 2 * Ns * steps
     step_s = ms / 1e3 / contour_steps
-    b_step = algorithmic_bytes(M, N, Nk)["step"]
-    print(json.dumps({
-        "metric": "MDE contour-steps/s at 128^3 Lmax=8", "value": round(contour_steps / (ms / 1e3), 2),
+    b_step = algorithmic_bytes(M, N, Nk)["step"] - (32 * M * Nk if fp32 else 0)
+    out = {
+        "metric": "MDE contour-steps/s at 128^3 Lmax=8" + (", FP32 half spectrum" if fp32 else ""),
+        "value": round(contour_steps / (ms / 1e3), 2),
         "unit": "contour-steps/s", "n_gpus": 1, "steps": steps, "ms_per_step": round(ms / steps, 2),
-        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "higher_is_better": True,
+        "dtype": "f64 arithmetic, f32 spectrum storage" if fp32 else "f64", "data": "synthetic",
         "config": {"workload": "MDE forward+backward sweep, 128^3, box 8.0, Lmax=8 (81 harmonics), "
                                "Ns=256, lambda=0.5, w = s_b (10 (sin X cos Y + sin Y cos Z + sin Z cos X) "
                                "+ 0.5 N(0,1) seed 2), s = (+,0,-); forward store checkpointed every 16 "
@@ -159,7 +171,14 @@ def bench_grid128(W, args):
                           "frac": round(b_step / step_s / 1e9 / PEAK_HBM_GBS, 4)},
         "kernels_avg_us": {k: round(1e3 * m / c, 1) if c else 0.0 for k, m, c in zip(KERNELS, kms, kcnt)},
         "Q": Q,
-    }))
+    }
+    if fp32:
+        _, _, _, phi64, Q64, F64 = runs[False]
+        out["fp64_value"] = round(contour_steps / (runs[False][0] / 1e3), 2)
+        out["error_vs_fp64"] = {"phi_max_rel": float(np.abs(phi - phi64).max() / np.abs(phi64).max()),
+                                "Q_rel": abs(Q - Q64) / Q64, "F_abs_kT": abs(F - F64),
+                                "F_fp64": F64, "chiN": 20.0}
+    print(json.dumps(out))
 
 
 def free_energy(w, phi, Q, chi, box):
