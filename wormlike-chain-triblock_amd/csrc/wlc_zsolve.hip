@@ -609,8 +609,7 @@ hipError_t zsolve_launch(const Geom& g, const SolveArgs& a, hipStream_t st) {
   if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
     // two workgroups per 4-kx tile
     const int blocks = 2 * g.nt * g.ny;
-    if (a.occ == 3) hipLaunchKernelGGL((k_zsolve64s<3, S>), dim3(blocks), dim3(256), 0, st, g, a);
-    else hipLaunchKernelGGL((k_zsolve64s<2, S>), dim3(blocks), dim3(256), 0, st, g, a);
+    hipLaunchKernelGGL((k_zsolve64s<2, S>), dim3(blocks), dim3(256), 0, st, g, a);
     return hipGetLastError();
   }
   if (g.nz == 128 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
