@@ -607,8 +607,12 @@ int validate(const wlc_params* p) {
 }
 
 int pick_tx(int nz) {
-  // kernel B: one thread per k-point, TX kx columns of nz points per workgroup
-  return nz >= 256 ? 1 : 256 / nz;
+  // generic kernel B: one thread per k-point, TX kx columns of nz points per workgroup;
+  // at most four columns (measured: 24^3 and 32^3 at Lmax 4, 48^3 at Lmax 6 run fastest
+  // with four) and at least one wave per workgroup
+  if (nz >= 256) return 1;
+  const int tx = std::min(4, 256 / nz);
+  return std::max(tx, (64 + nz - 1) / nz);
 }
 
 // backward sweep concurrent with the queued forward sweep (see the handle notes)
@@ -707,6 +711,10 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     st += p->Nblk[b];
     h->Mmax = std::max(h->Mmax, h->Mb[b]);
     h->TX[b] = pick_tx(p->nz);
+    if (const char* tx = std::getenv("WLC_ZSOLVE_TX")) {  // measurement override
+      const int v = std::atoi(tx);
+      if (v >= 1 && v * p->nz <= 256) h->TX[b] = v;
+    }
     std::vector<double> cw;
     contour_weights(p->Nblk[b], h->ds, &cw);
     h->cw.push_back(cw);
