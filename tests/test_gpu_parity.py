@@ -44,6 +44,8 @@ CASES = [
     ((24, 4, 6), (2.0, 2.0, 5.0), (6, 7), (0, 2), (1.0, 0.0), 0),
     ((2, 2, 2), (1.0, 1.0, 1.0), (2, 1, 2), (8, 5, 6), (0.5, 1.0, 3.0), 0),
     ((16, 4, 4), (2.0, 2.0, 2.0), (3, 3, 3), (6, 6, 6), (1.0, 1.0, 1.0), 0),
+    # nz = 64 at Lmax 8: the one-wave kernel with all 81 harmonics of a column
+    ((64, 4, 4), (3.0, 2.0, 2.0), (3, 4), (8, 4), (0.5, 1.0), 0),
     # nz = 64 with Lmax 5..7: the split z-solve kernel (both parts, crossing chains)
     ((64, 4, 6), (3.0, 2.0, 2.5), (4, 3, 4), (6, 5, 7), (1.0, 0.7, 1.3), 0),
     # ... with nx/2 + 1 = 5 columns: a half-padded and an all-padding workgroup in the last tile
