@@ -48,10 +48,12 @@ typedef struct wlc_params {
   double lambda[WLC_MAX_BLOCKS];  /* angular coefficient Loa/(2 kappa) per block */
   int keep_qstar;                 /* 1: keep every backward slice (for wlc_mde_get_q) */
   int ckpt_every;                 /* forward store: 0 = every slice when it fits in device
-                                     memory, else checkpoints every 16 steps; K >= 3 =
-                                     checkpoints every K steps, segments recomputed during
-                                     the backward sweep (no counterpart in the reference,
-                                     which stores every slice, init_array_2dcomp_mod.f90:47-51) */
+                                     memory, else segments of 16 steps: the lowest segments
+                                     that fit are kept whole, the others checkpointed and
+                                     recomputed during the backward sweep; K >= 3 = every
+                                     segment of K steps checkpointed (no counterpart in the
+                                     reference, which stores every slice,
+                                     init_array_2dcomp_mod.f90:47-51) */
   int spectrum_fp32;              /* 1: store the half spectrum (the FFT / solve intermediate)
                                      as FP32, every arithmetic step stays FP64; halves the
                                      spectrum traffic at ~1e-7 rounding per store.  No
@@ -103,6 +105,11 @@ int wlc_mde_get_q(wlc_handle* h, int dir, int s, double* q_host);
 
 /* harmonics held by slice s (M of the block that contains contour point s) */
 int wlc_mde_slice_M(const wlc_handle* h, int s);
+
+/* forward slices 0 .. n-1 kept whole by the forward sweep: Ns + 1 with the full store;
+ * in checkpoint mode the slices of the lowest segments that fitted beside the checkpoints
+ * (0 when every segment is recomputed).  No counterpart in the reference. */
+int wlc_mde_forward_stored(const wlc_handle* h);
 
 /* block index (0-based) that owns contour point s */
 int wlc_mde_block_of(const wlc_handle* h, int s);

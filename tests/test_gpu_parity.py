@@ -237,16 +237,20 @@ def test_concurrent_sweeps_match_sequential(wlc, monkeypatch, keep):
     assert abs(t1 - t0) / abs(t0) < 1e-13 and _rel(p1, p0) < 1e-13
 
 
-@pytest.mark.parametrize("K", [3, 5, 8, 40])
-def test_checkpointed_forward_store_matches_full_store(wlc, monkeypatch, K):
+@pytest.mark.parametrize("K,stored", [(3, 0), (5, 0), (8, 0), (40, 0), (3, 2), (5, 1), (8, 2)])
+def test_checkpointed_forward_store_matches_full_store(wlc, monkeypatch, K, stored):
     """With ckpt_every = K the forward sweep keeps only segment-boundary checkpoints and the
     backward sweep recomputes each segment (on its own stream, double-buffered) before its
-    densities: phi, Q, totden, q(Ns) and q*(0) are bit-identical to the full store."""
+    densities; with `stored` > 0 the lowest segments are kept whole (the hybrid store of the
+    automatic mode) and only the ones above are recomputed.  phi, Q, totden, q(Ns), q*(0)
+    and the stored forward slices are bit-identical to the full store."""
     monkeypatch.setenv("WLC_CONCURRENT", "0")
+    monkeypatch.setenv("WLC_STORED_SEGMENTS", str(stored))
     shape, box, Nblk, L, lam = (8, 16, 16), (2.0, 3.0, 3.0), (7, 9, 7), (3, 4, 2), (1.0, 0.6, 1.5)
     rng = np.random.default_rng(21)
     w = 1.5 * rng.standard_normal((3,) + shape)
     res = []
+    top = stored * K  # last stored forward slice
     for k in (0, K):
         g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, ckpt_every=k)
         g.set_w(w)
@@ -254,15 +258,17 @@ def test_checkpointed_forward_store_matches_full_store(wlc, monkeypatch, K):
             g.MDE_q()
             g.MDE_qstar()
             phi, Q, td = g.density()
-        res.append((phi, Q, td, g.get_q(+1, sum(Nblk)), g.get_q(-1, 0)))
+        res.append((phi, Q, td, g.get_q(+1, sum(Nblk)), g.get_q(-1, 0),
+                    [g.get_q(+1, s) for s in range(top + 1)] if stored else []))
         if k:
             with pytest.raises(wlc.WlcError) as e:
-                g.get_q(+1, 5)
+                g.get_q(+1, top + 1)
             assert e.value.code == -4
         g.close()
-    (p0, Q0, t0, q0, s0), (p1, Q1, t1, q1, s1) = res
+    (p0, Q0, t0, q0, s0, f0), (p1, Q1, t1, q1, s1, f1) = res
     assert np.array_equal(q0, q1) and np.array_equal(s0, s1)
     assert Q0 == Q1 and t0 == t1 and np.array_equal(p0, p1)
+    assert all(np.array_equal(a, b) for a, b in zip(f0, f1))
 
 
 @pytest.mark.parametrize("mode", [1, 2])

@@ -123,6 +123,11 @@ struct wlc_handle {
   // (on st2, after the last density of the segment it held) marks it reusable.  The
   // recomputed slices are bit-identical to the first forward sweep's.
   int ck = 0, nseg = 0;
+  // Hybrid store: the lowest nstored segments (slices 0 .. nstored K) are kept whole in
+  // `store` by the forward sweep and read in place by the backward densities; only the
+  // segments above are checkpointed and recomputed.
+  int nstored = 0;
+  double* store = nullptr;
   double* fr = nullptr;
   double* ckpt = nullptr;
   double* segbuf[2] = {nullptr, nullptr};
@@ -168,7 +173,10 @@ struct SweepCtx {
 long long slot_size(const wlc_handle* h) { return (long long)h->Mmax * h->hs; }
 
 double* fwd_slice(const wlc_handle* h, const FwdLoc& fl, int s) {
-  if (fl.mode == 1) return h->fr + (long long)(s & 3) * slot_size(h);
+  if (fl.mode == 1) {
+    if (h->nstored > 0 && s <= h->nstored * h->ck) return h->store + (long long)s * slot_size(h);
+    return h->fr + (long long)(s & 3) * slot_size(h);
+  }
   if (fl.mode == 2) return fl.base + (long long)(s - fl.lo) * slot_size(h);
   return h->q + h->off[s];
 }
@@ -402,8 +410,18 @@ int sweep(wlc_handle* h, const SweepCtx& sw, int first, int last, bool dens) {
   return WLC_OK;
 }
 
+// first segment restarted from a checkpoint (segment nstored restarts from the store)
+int first_ckpt_segment(const wlc_handle* h) { return h->nstored > 0 ? h->nstored + 1 : 1; }
+
 double* ckpt_slot(const wlc_handle* h, int j, int t) {
-  return h->ckpt + ((long long)(j - 1) * 3 + t) * slot_size(h);
+  return h->ckpt + ((long long)(j - first_ckpt_segment(h)) * 3 + t) * slot_size(h);
+}
+
+// restart slice jK - 2 + t of segment j >= 1
+const double* restart_slice(const wlc_handle* h, int j, int t) {
+  if (h->nstored > 0 && j == h->nstored)
+    return h->store + (long long)(j * h->ck - 2 + t) * slot_size(h);
+  return ckpt_slot(h, j, t);
 }
 
 // after forward slice s on the ring: keep the restart history of every segment boundary
@@ -413,7 +431,7 @@ int save_checkpoint(wlc_handle* h, const SweepCtx& sw, int s) {
   const size_t bytes = sizeof(double) * (size_t)h->Mb[block_of(h, s)] * h->hs;
   // j = floor((s + 2) / K) gives jK - 2 <= s < (j + 1)K - 2: only s <= jK is a restart slice
   const int j = (s + 2) / K;
-  if (j >= 1 && j <= h->nseg - 1 && s <= j * K)
+  if (j >= first_ckpt_segment(h) && j <= h->nseg - 1 && s <= j * K)
     HIPCHK(hipMemcpyAsync(ckpt_slot(h, j, s - (j * K - 2)), fwd_slice(h, sw.fl, s), bytes,
                           hipMemcpyDeviceToDevice, sw.st));
   if (s == h->Ns)
@@ -442,7 +460,7 @@ int recompute_segment(wlc_handle* h, int j) {
     for (int t = 0; t < 3; ++t) {
       const int s = j * K - 2 + t;
       const size_t bytes = sizeof(double) * (size_t)h->Mb[block_of(h, s)] * h->hs;
-      HIPCHK(hipMemcpyAsync(fwd_slice(h, sw.fl, s), ckpt_slot(h, j, t), bytes,
+      HIPCHK(hipMemcpyAsync(fwd_slice(h, sw.fl, s), restart_slice(h, j, t), bytes,
                             hipMemcpyDeviceToDevice, h->str));
     }
   }
@@ -496,17 +514,17 @@ int density_at(wlc_handle* h, int s) {
     // the buffer of the segment above and queues the recomputation two segments down
     const int j = seg_of(h, s);
     if (j != h->dens_seg) {
-      if (h->dens_seg >= 0) {
+      if (h->dens_seg >= h->nstored) {
         HIPCHK(hipEventRecord(h->ev_seg_free[h->dens_seg & 1], h->st2));
-        if (j - 1 >= 0) {
+        if (j - 1 >= h->nstored) {
           int rc = recompute_segment(h, j - 1);
           if (rc) return rc;
         }
       }
-      HIPCHK(hipStreamWaitEvent(h->st2, h->ev_seg_ready[j & 1], 0));
+      if (j >= h->nstored) HIPCHK(hipStreamWaitEvent(h->st2, h->ev_seg_ready[j & 1], 0));
       h->dens_seg = j;
     }
-    qf = fwd_slice(h, seg_loc(h, j), s);
+    qf = j < h->nstored ? h->store + (long long)s * slot_size(h) : fwd_slice(h, seg_loc(h, j), s);
   }
   HIPCHK(hipEventRecord(h->ev_step, h->st));
   HIPCHK(hipStreamWaitEvent(h->st2, h->ev_step, 0));
@@ -561,6 +579,7 @@ void free_handle(wlc_handle* h) {
   (void)hipFree(h->specb);
   (void)hipFree(h->rho2);
   (void)hipFree(h->fr);
+  (void)hipFree(h->store);
   (void)hipFree(h->ckpt);
   (void)hipFree(h->segbuf[0]);
   (void)hipFree(h->segbuf[1]);
@@ -757,13 +776,36 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     const size_t base = 4 * slot_bytes + spec_bytes + 4 * field_bytes + (size_t(2) << 30);
+    bool automatic = false;
     if (p->ckpt_every > 0) h->ck = p->ckpt_every;
-    else if (store_bytes + base > free_b) h->ck = 16;
-    if (h->ck) h->nseg = (h->Ns + h->ck - 1) / h->ck;
+    else if (store_bytes + base > free_b) h->ck = 16, automatic = true;
+    if (h->ck) {
+      h->nseg = (h->Ns + h->ck - 1) / h->ck;
+      // automatic mode: keep as many of the lowest segments whole as fit beside the
+      // checkpoint buffers (the last segment is always recomputed); an explicit ckpt_every
+      // checkpoints every segment unless WLC_STORED_SEGMENTS asks for a mix (tests)
+      const size_t ck_fixed = base + (size_t)(4 + 1 + 2 * (h->ck + 3)) * slot_bytes + spec_bytes;
+      if (automatic) {
+        for (int n = h->nseg - 1; n >= 1; --n) {
+          const size_t need = ck_fixed + (size_t)(n * h->ck + 1) * slot_bytes +
+                              (size_t)(h->nseg - (n + 1)) * 3 * slot_bytes;
+          if (need <= free_b) {
+            h->nstored = n;
+            break;
+          }
+        }
+      } else if (const char* ns = std::getenv("WLC_STORED_SEGMENTS")) {
+        h->nstored = std::max(0, std::min(std::atoi(ns), h->nseg - 1));
+      }
+    }
   }
   if (h->ck) {
     ALLOC(h->fr, 4 * slot_bytes);
-    if (h->nseg > 1) ALLOC(h->ckpt, (size_t)(h->nseg - 1) * 3 * slot_bytes);
+    if (h->nstored > 0) ALLOC(h->store, (size_t)(h->nstored * h->ck + 1) * slot_bytes);
+    {
+      const int nck = h->nseg - (h->nstored > 0 ? h->nstored + 1 : 1);
+      if (nck > 0) ALLOC(h->ckpt, (size_t)nck * 3 * slot_bytes);
+    }
     for (int b = 0; b < 2; ++b) ALLOC(h->segbuf[b], (size_t)(h->ck + 3) * slot_bytes);
     ALLOC(h->qlast, slot_bytes);
     ALLOC(h->specr, spec_bytes);
@@ -922,9 +964,9 @@ int wlc_mde_backward(wlc_handle* h) {
     HIPCHK(hipStreamWaitEvent(h->str, h->ev_step, 0));
     h->dens_seg = -1;
     h->seg_busy[0] = h->seg_busy[1] = false;
-    const int J = h->nseg - 1;
+    const int J = h->nseg - 1;  // >= nstored: the store never holds the last slice
     rc = recompute_segment(h, J);
-    if (rc == WLC_OK && J >= 1) rc = recompute_segment(h, J - 1);
+    if (rc == WLC_OK && J - 1 >= h->nstored) rc = recompute_segment(h, J - 1);
   }
   if (rc == WLC_OK && dens) rc = density_at(h, Ns);
   if (rc == WLC_OK) rc = sweep(h, main_sweep(h, -1), Ns - 1, 0, dens);
@@ -992,8 +1034,9 @@ int wlc_mde_get_q(wlc_handle* h, int dir, int s, double* q_host) {
   const double* src;
   if (dir > 0) {
     if (!h->fwd_done) return fail(WLC_ESTATE, "no forward sweep");
-    if (h->ck && s != h->Ns) return fail(WLC_ESTATE, "forward slices other than Ns are not stored in checkpoint mode");
-    src = h->ck ? h->qlast : h->q + h->off[s];
+    if (h->ck && s != h->Ns && (h->nstored == 0 || s > h->nstored * h->ck))
+      return fail(WLC_ESTATE, "this forward slice is not stored in checkpoint mode");
+    src = !h->ck ? h->q + h->off[s] : s == h->Ns ? h->qlast : h->store + (long long)s * slot_size(h);
   } else {
     if (!h->p.keep_qstar && s != 0) return fail(WLC_ESTATE, "backward slices other than 0 need keep_qstar");
     src = qs_slot(h, s);
@@ -1007,6 +1050,12 @@ int wlc_mde_get_q(wlc_handle* h, int dir, int s, double* q_host) {
 int wlc_mde_slice_M(const wlc_handle* h, int s) {
   if (!h || s < 0 || s > h->Ns) return WLC_EINVAL;
   return h->Mb[block_of(h, s)];
+}
+
+int wlc_mde_forward_stored(const wlc_handle* h) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  if (!h->ck) return h->Ns + 1;
+  return h->nstored > 0 ? h->nstored * h->ck + 1 : 0;
 }
 
 int wlc_mde_block_of(const wlc_handle* h, int s) {

@@ -50,7 +50,7 @@ EXPORTS = [
     "wlc_mde_get_q", "wlc_mde_slice_M", "wlc_mde_block_of", "wlc_mde_sync",
     "wlc_mde_stream", "wlc_mde_set_timing", "wlc_mde_get_timing", "wlc_mde_event_record",
     "wlc_mde_event_elapsed", "wlc_mde_destroy", "wlc_mde_last_error", "wlc_test_solve",
-    "wlc_mde_set_spectrum_fp32",
+    "wlc_mde_set_spectrum_fp32", "wlc_mde_forward_stored",
 ]
 
 _lib = None
@@ -78,6 +78,7 @@ def load(path=LIB_PATH):
     L.wlc_mde_get_q.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.wlc_mde_slice_M.argtypes = [vp, ctypes.c_int]
     L.wlc_mde_block_of.argtypes = [vp, ctypes.c_int]
+    L.wlc_mde_forward_stored.argtypes = [vp]
     L.wlc_mde_sync.argtypes = [vp]
     L.wlc_mde_stream.argtypes = [vp]
     L.wlc_mde_stream.restype = vp
@@ -207,6 +208,13 @@ class WormlikeMDE:
         _check(_lib.wlc_mde_sync(self.h))
 
     # --- measurement ---------------------------------------------------------------
+    def forward_stored(self):
+        """Number of leading forward slices kept whole (Ns + 1 with the full store)."""
+        n = _lib.wlc_mde_forward_stored(self.h)
+        if n < 0:
+            _check(n)
+        return n
+
     def set_spectrum_fp32(self, on):
         _check(_lib.wlc_mde_set_spectrum_fp32(self.h, 1 if on else 0))
 
