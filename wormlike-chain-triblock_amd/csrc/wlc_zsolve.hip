@@ -219,8 +219,8 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
 // nz = 64, 5 <= Lmax <= 7: the harmonics of a column split over two waves by degree
 // ---------------------------------------------------------------------------
 // Lo holds l = 0..kLS, Hi holds l = kLS+1..L; each wave keeps only its part in registers,
-// which allows two to three waves per SIMD instead of one (at Lmax = 8 the larger part
-// alone fills a SIMD, so Lmax = 8 keeps the one-wave kernel).  Workgroup = two kx columns of a
+// which allows two waves per SIMD instead of one (at Lmax = 8 the larger part alone does
+// not fit two waves per SIMD, so Lmax = 8 keeps the one-wave kernel).  Workgroup = two kx columns of a
 // 4-kx tile (waves: column c = w >> 1, part = w & 1).  The two workgroups of a tile run
 // back to back on one XCD (block order below), so each tile's cache lines are fetched once.
 constexpr int kLS = kL >= 7 ? 5 : (kL == 6 ? 4 : 3);
