@@ -46,9 +46,10 @@ CASES = [
     ((16, 4, 4), (2.0, 2.0, 2.0), (3, 3, 3), (6, 6, 6), (1.0, 1.0, 1.0), 0),
     # nz = 64 at Lmax 8: the one-wave kernel with all 81 harmonics of a column
     ((64, 4, 4), (3.0, 2.0, 2.0), (3, 4), (8, 4), (0.5, 1.0), 0),
-    # nz = 64 with Lmax 5..7: the split z-solve kernel (both parts, crossing chains)
+    # nz = 64 with Lmax 5..7: the split z-solve kernel (both parts, crossing chains), one
+    # 4-kx tile per workgroup
     ((64, 4, 6), (3.0, 2.0, 2.5), (4, 3, 4), (6, 5, 7), (1.0, 0.7, 1.3), 0),
-    # ... with nx/2 + 1 = 5 columns: a half-padded and an all-padding workgroup in the last tile
+    # ... with nx/2 + 1 = 5 columns: a partial last tile (one real column of four)
     ((64, 6, 8), (3.0, 2.0, 4.0), (3, 4, 3), (7, 6, 5), (0.9, 1.1, 0.6), 0),
     # 64 x 64 planes (register-resident plane kernels), basis shrinks then grows
     ((4, 64, 64), (2.0, 5.0, 4.0), (3, 4, 3), (3, 2, 3), (1.0, 0.5, 2.0), 0),
@@ -192,12 +193,14 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape):
     """Lmax 5..7 at nz = 64 and 128 runs the z solve with the harmonics of a column split
     over two waves (per z half at nz = 128); the unsplit kernel computes the same step
     (equal up to rounding: the split kernel passes k = 0 through identity-angle rotations
-    instead of a diagonal shortcut).  nx/2 + 1 = 5 leaves padding columns in the last tile."""
+    instead of a diagonal shortcut).  nx/2 + 1 = 5 leaves padding columns in the last tile.
+    At nz = 64 the whole-tile split kernel (the default) and the two-column split kernel
+    run the same operations (equal up to the compiler's multiply-add contraction)."""
     box, Nblk, L, lam = (3.0, 2.0, 5.0), (3, 4, 3), (5, 7, 6), (1.2, 0.6, 1.0)
     rng = np.random.default_rng(31)
     w = 1.5 * rng.standard_normal((3,) + shape)
     out = []
-    for split in ("0", "1"):
+    for split in ("0", "1", "2") if shape[0] == 64 else ("0", "1"):
         monkeypatch.setenv("WLC_ZSOLVE_SPLIT", split)
         g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, keep_qstar=True)
         g.set_w(w)
@@ -206,9 +209,14 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape):
         phi, Q, _ = g.density()
         out.append((g.get_q(+1, sum(Nblk)), g.get_q(-1, 0), phi, Q))
         g.close()
-    (q0, s0, p0, Q0), (q1, s1, p1, Q1) = out
+    (q0, s0, p0, Q0), (q1, s1, p1, Q1) = out[:2]
     assert _rel(q1, q0) < 1e-13 and _rel(s1, s0) < 1e-13 and _rel(p1, p0) < 1e-13
     assert abs(Q1 - Q0) / abs(Q0) < 1e-13
+    if len(out) == 3:
+        q2, s2, p2, Q2 = out[2]
+        print("tile vs two-column kernel:", _rel(q2, q1), _rel(s2, s1), _rel(p2, p1), flush=True)
+        assert _rel(q2, q1) < 1e-13 and _rel(s2, s1) < 1e-13 and _rel(p2, p1) < 1e-13
+        assert abs(Q2 - Q1) / abs(Q1) < 1e-13
 
 
 @pytest.mark.parametrize("keep", [False, True])

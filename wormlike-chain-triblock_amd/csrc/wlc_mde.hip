@@ -140,7 +140,7 @@ struct wlc_handle {
   bool have_w = false, fwd_done = false, bwd_done = false;
   int generic_zsolve = 0;
   int zs_occ = 1;
-  int zs_split = 1;  // split z-solve kernel for Lmax >= 5 (nz = 64)
+  int zs_split = 2;  // Lmax 5-7 harmonics split by degree: 1 two-column, 2 whole-tile (nz = 64)
   int fuse = 1;  // kernel C of a step fused with kernel A of the next
   // timing
   bool timing = false;
@@ -697,7 +697,10 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   h->p = *p;
   if (const char* ge = std::getenv("WLC_GENERIC_ZSOLVE")) h->generic_zsolve = std::atoi(ge) != 0;
   if (const char* oc = std::getenv("WLC_ZSOLVE_OCC")) h->zs_occ = std::atoi(oc) == 2 ? 2 : 1;
-  if (const char* sp = std::getenv("WLC_ZSOLVE_SPLIT")) h->zs_split = std::atoi(sp) != 0;
+  if (const char* sp = std::getenv("WLC_ZSOLVE_SPLIT")) {
+    const int v = std::atoi(sp);
+    h->zs_split = (v == 1 || v == 2) ? v : 0;
+  }
   if (const char* fu = std::getenv("WLC_FUSE")) h->fuse = std::atoi(fu) != 0;
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;

@@ -377,6 +377,177 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// nz = 64, 5 <= Lmax <= 7: one 4-kx tile per workgroup, harmonics split by degree
+// ---------------------------------------------------------------------------
+// Eight waves: wave w = 4 p + c holds part p (0: l <= kLS, 1: l > kLS) of column
+// kx = 4 t + c.  The four waves of a part move each of their harmonics' 4 KB tiles
+// (64 z x 4 kx, contiguous) between HBM and their four LDS regions with coalesced
+// 16-byte accesses (thread i of the part: z = i >> 2, column i & 3), eight harmonics per
+// batch, the next batch's loads in flight while the current batch is transformed.  Both
+// parts run kNBt batches (the smaller part idles through its last ones), so every wave
+// passes the same barriers.  Columns beyond nx/2 (the padding of a partial last tile)
+// are neither read, transformed nor written.
+constexpr int kNBt = (kNV + kHB - 1) / kHB;
+constexpr int kTileLds = 2 * kKxTile * kRegionS;  // dynamic LDS, double2 units
+
+template <class S>
+__device__ __forceinline__ void zt_forward(double2 (&v)[kNV], double2* glds, double2* wl, int lane,
+                                           int gi, int nh, bool run, const double2 (&tw)[8],
+                                           const S* __restrict__ hp, long long istride, bool io) {
+  const int h = lane >> 3, j = lane & 7;
+  const int tz = gi >> 2, tc = gi & 3;
+  double2 nxt[kHB];
+#pragma unroll
+  for (int q = 0; q < kHB; ++q)
+    nxt[q] = (io && q < nh) ? spec_ld(hp + (long long)q * istride + gi) : make_double2(0.0, 0.0);
+#pragma unroll
+  for (int bb = 0; bb < kNBt; ++bb) {
+    const int b0 = bb * kHB;
+    __syncthreads();  // every wave of the part has read the previous batch out of its region
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (b0 + q < nh) glds[tc * kRegionS + q * kPadBatch + pad8(tz)] = nxt[q];
+    if (bb + 1 < kNBt) {
+#pragma unroll
+      for (int q = 0; q < kHB; ++q) {
+        const int hq = b0 + kHB + q;
+        nxt[q] = (io && hq < nh) ? spec_ld(hp + (long long)hq * istride + gi) : make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    if (run && b0 < nh) {
+      double2 u[8];
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
+      Dft<8, -1>::run(u);
+#pragma unroll
+      for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<-1>(u[k2], tw[k2]);
+      wave_sync();
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
+      wave_sync();
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
+      Dft<8, -1>::run(u);
+      wave_sync();
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < kHB; ++q)
+        if (b0 + q < kNV && b0 + q < nh) v[b0 + q] = wl[q * kPadBatch + pad8(lane)];
+    }
+  }
+}
+
+template <class S>
+__device__ __forceinline__ void zt_inverse(const double2 (&v)[kNV], double2* glds, double2* wl,
+                                           int lane, int gi, int nh, bool run,
+                                           const double2 (&tw)[8], S* __restrict__ hp,
+                                           long long istride, bool io) {
+  const int h = lane >> 3, j = lane & 7;
+  const int tz = gi >> 2, tc = gi & 3;
+#pragma unroll
+  for (int bb = 0; bb < kNBt; ++bb) {
+    const int b0 = bb * kHB;
+    __syncthreads();  // the previous batch's tiles have been read out of the regions
+    if (run && b0 < nh) {
+#pragma unroll
+      for (int q = 0; q < kHB; ++q)
+        if (b0 + q < kNV && b0 + q < nh) wl[q * kPadBatch + pad8(lane)] = v[b0 + q];
+      wave_sync();
+      double2 u[8];
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
+      Dft<8, +1>::run(u);
+#pragma unroll
+      for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<+1>(u[k2], tw[k2]);
+      wave_sync();
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
+      wave_sync();
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
+      Dft<8, +1>::run(u);
+      wave_sync();
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kHB; ++q)
+      if (io && b0 + q < nh)
+        spec_st(hp + (long long)(b0 + q) * istride + gi, glds[tc * kRegionS + q * kPadBatch + pad8(tz)]);
+  }
+}
+
+template <class S>
+__global__ void __launch_bounds__(512) k_zsolve64t(Geom g, SolveArgs a) {
+  extern __shared__ double2 lds[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int part = wv >> 2, c = wv & 3, gi = threadIdx.x & 255;
+  const int ny = g.ny, nxh = g.nxh, nt = g.nt;
+  // tiles with four real columns first; the partial last tile of every ky row (when
+  // nx/2 + 1 is not a multiple of 4) last, so the final, partly filled round of
+  // workgroups holds the cheap ones
+  const int nfull = nxh / kKxTile;
+  const int b = blockIdx.x;
+  int t4, ky;
+  if (b < nfull * ny) {
+    t4 = b % nfull;
+    ky = b / nfull;
+  } else {
+    t4 = nfull;
+    ky = b - nfull * ny;
+  }
+  const int kx = t4 * kKxTile + c;
+  const bool real_col = kx < nxh;                       // wave-uniform
+  const bool io = t4 * kKxTile + (gi & 3) < nxh;        // the column this thread moves
+  const bool active = a.solve && real_col;
+  const long long istride = spec_index(1, 0, 0, 0, ny, nt, 64);
+  const int nh = part ? PartHi::NH : PartLo::NH;
+  const int hb0 = part ? PartHi::B0 : PartLo::B0;
+  S* hp = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, 64) +
+          (long long)hb0 * istride;
+  double2* glds = lds + part * (kKxTile * kRegionS);
+  double2* wl = glds + c * kRegionS;
+  double2* xch = lds + c * kRegionS;  // the Lo wave's region of column c
+  double2 tw[8];
+  {
+    const int jt = lane & 7;
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) tw[k2] = g.az.tw[jt * k2];
+  }
+  const KFrame f(signed_k(kx, g.nx) * a.kfac[0], signed_k(ky, ny) * a.kfac[1],
+                 signed_k(lane, 64) * a.kfac[2], a.sigma, a.ds);
+  double2 v[kNV];
+  // phase 1: load, forward z transform, rotate into the k frame; Lo: forward elimination
+  zt_forward(v, glds, wl, lane, gi, nh, real_col, tw, hp, istride, io);
+  if (active) {
+    if (part == 0) {
+      PartLo::to_frame(v, f);
+      tri_lower_forward<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+    } else {
+      PartHi::to_frame(v, f);
+    }
+  }
+  __syncthreads();
+  // phase 2: Hi finishes every chain
+  if (part == 1 && active) tri_upper<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+  __syncthreads();
+  // phase 3: Lo back substitution; rotate back, inverse z transform, store
+  if (active) {
+    if (part == 0) {
+      tri_lower_back<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+      PartLo::from_frame(v, f);
+    } else {
+      PartHi::from_frame(v, f);
+    }
+  }
+  zt_inverse(v, glds, wl, lane, gi, nh, real_col, tw, hp, istride, io);
+}
+
+// ---------------------------------------------------------------------------
 // nz = 128: two waves per (kx, ky) column
 // ---------------------------------------------------------------------------
 // Wave p of a column loads z = m + 64 p (lane m).  Forward: one radix-2 decimation-in-
@@ -606,6 +777,15 @@ namespace {
 template <class S>
 hipError_t zsolve_launch(const Geom& g, const SolveArgs& a, hipStream_t st) {
   if (a.M != kM) return hipErrorInvalidValue;
+  if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split == 2) {
+    // one workgroup of eight waves per 4-kx tile
+    const size_t lds = sizeof(double2) * (size_t)kTileLds;
+    hipError_t e = hipFuncSetAttribute((const void*)k_zsolve64t<S>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_zsolve64t<S>), dim3(g.nt * g.ny), dim3(512), lds, st, g, a);
+    return hipGetLastError();
+  }
   if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
     // two workgroups per 4-kx tile
     const int blocks = 2 * g.nt * g.ny;
