@@ -211,3 +211,59 @@ def test_fortran_fp32_spectrum_then_fp64(wlc, tmp_path):
     assert np.abs(got[:, 1] - ref[:, 1]).max() < 1e-6       # F, kT
     assert np.abs(got[:, 2] - ref[:, 2]).max() < 1e-6       # ln Q
     assert np.abs(wgot - wref).max() < 1e-6 * np.abs(wref).max()
+
+
+def _list_directed(path):
+    """Rows of a list-directed file: numbers as floats, anything else as text."""
+    rows = []
+    for ln in open(path).read().splitlines():
+        row = []
+        for tok in ln.split():
+            try:
+                row.append(float(tok))
+            except ValueError:
+                row.append(tok)
+        rows.append(row)
+    return rows
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran binding not built (no flang)")
+def test_fortran_run_logs(wlc, tmp_path):
+    """--logs writes the reference's run logs (SCFT_dump.f90:35-56, :183-197): per iteration
+    error.dat (residuals) and FreeEG.dat (F, ln Q, field energy, F minus the disordered
+    sum_{a<b} chi_ab f_a f_b); once converged result.txt and result.dat.  The values agree
+    with the driver's own output lines; F = -ln Q + field energy."""
+    rng = np.random.default_rng(9)
+    w = 0.5 * rng.standard_normal((3,) + CASE["shape"])
+    f = np.array(CASE["Nblk"]) / sum(CASE["Nblk"])
+    fh = sum(CHI[a, b] * f[a] * f[b] for a in range(3) for b in range(a + 1, 3))
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    rows, _, _ = run_driver(a, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05, tol=1e-12,
+                            flags=["--logs"])
+    err = np.array(_list_directed(a / "error.dat"))
+    assert err.shape == (3, 4)
+    assert np.array_equal(err[:, 0], rows[:, 0])
+    assert np.abs(err[:, 1:] - rows[:, 4:7]).max() <= 1e-13 * np.abs(rows[:, 4:7]).max()
+    fe = _list_directed(a / "FreeEG.dat")
+    assert len(fe) == 12
+    for it in range(3):
+        l1, l2, l3, l4 = fe[4 * it:4 * it + 4]
+        F, lnQ = rows[it, 1], rows[it, 2]
+        assert l1[0] == it + 1 and abs(l1[1] - F) < 1e-12 and abs(l1[2] - lnQ) < 1e-12
+        assert abs(l1[3] - (F + lnQ)) < 1e-11                      # F = -ln Q + field energy
+        assert l2[0] == it + 1 and l2[1] == CASE["box"][2] and l2[2] == "FE-FH"
+        assert abs(l2[3] - (F - fh)) < 1e-11
+        assert l3[0] == "dx,dy" and l4[0] == "box"
+    assert not (a / "result.dat").exists() and (a / "result.txt").read_text() == ""
+    # a tolerance every residual meets: converged after the first iteration
+    rows, _, trailer = run_driver(b, **CASE, chi=CHI, w=w, niter=3, method=0, lam_s=0.05,
+                                  tol=1e6, flags=["--logs"])
+    assert trailer[1:5] == ["iterations", "1", "converged", "1"]
+    fe = _list_directed(b / "FreeEG.dat")
+    assert len(fe) == 2 and len(fe[0]) == 5 and abs(fe[0][4] - (rows[0, 1] - fh)) < 1e-11
+    res = _list_directed(b / "result.dat")
+    assert [r[0] for r in res] == ["FREE_ENERGY", "FREE_ENERGY-FH", "box", "converge"]
+    assert abs(res[0][1] - rows[0, 1]) < 1e-12 and res[3][1] == "T"
+    assert (b / "result.txt").read_text() == (b / "result.dat").read_text()

@@ -15,6 +15,14 @@
 !   --split-operator-slot  set up and solve through the split-operator call slot
 !            (split_operator_MDE_init / AB_diblock_sph_driver) instead of MDE_setup /
 !            MDE_q / MDE_qstar (same results: the slot runs the same solve)
+!   --logs   write the reference's run logs in the working directory (created empty at
+!            start as initialize_mod.f90:296-306 does; SCFT_dump.f90:35-56 and :183-197
+!            append after every iteration): error.dat (iteration, residual per species),
+!            FreeEG.dat (iteration, F, ln Q, field energy; iteration, box x, "FE-FH",
+!            F - sum_{a<b} chi_ab f_a f_b; then "dx,dy" and "box size" lines until
+!            converged), and when converged result.txt (appended) and result.dat
+!            (replaced): FREE_ENERGY, FREE_ENERGY-FH, box size, converge (records up to
+!            1024 characters, so no list-directed line wraps)
 ! input (stream, native endian):
 !   int32  nx, ny, nz, nblk, Ns, Nblk(3), Lmax(3), max_iter, method (0 simple mixing only,
 !          1 simple then Anderson), n_simple, n_hist, Anderson form (0 reference, 1 mixing
@@ -32,7 +40,7 @@ program scft_driver
   implicit none
   character(len=1024) :: fin, fout, fw, arg
   integer :: u, hdr(16), max_iter, method, it, a, b, n, nargs, done, i, npos, stat
-  logical :: read_w, dump_w, sph_slot, fp32_on, next_is_tol
+  logical :: read_w, dump_w, sph_slot, fp32_on, next_is_tol, logs
   real(8) :: fp32_tol
   integer(8) :: c0, c1, rate
   real(8) :: box(3), lam(3), chi9(9), chi(3, 3), f(3), tol, mix(2)
@@ -40,7 +48,7 @@ program scft_driver
 
   nargs = command_argument_count()
   fin = ''; fout = ''; fw = ''
-  read_w = .false.; dump_w = .false.; sph_slot = .false.
+  read_w = .false.; dump_w = .false.; sph_slot = .false.; logs = .false.
   fp32_on = .false.; next_is_tol = .false.; fp32_tol = 0d0
   npos = 0
   do i = 1, nargs
@@ -57,6 +65,8 @@ program scft_driver
         dump_w = .true.
      else if (trim(arg) == '--split-operator-slot') then
         sph_slot = .true.
+     else if (trim(arg) == '--logs') then
+        logs = .true.
      else
         npos = npos + 1
         if (npos == 1) fin = arg
@@ -66,7 +76,7 @@ program scft_driver
   end do
   if (npos < 2 .or. npos > 3 .or. next_is_tol) then
      write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin] [--readW] [--dumpW] &
-          &[--spectrum-fp32 <tol>] [--split-operator-slot]'
+          &[--spectrum-fp32 <tol>] [--split-operator-slot] [--logs]'
      error stop 2
   end if
 
@@ -109,6 +119,7 @@ program scft_driver
      f(a) = dble(Ns_blk(a)) / dble(Nmax)
   end do
 
+  if (logs) call logs_create()
   open(newunit=u, file=trim(fout), status='replace', action='write')
   converged = .false.
   done = 0
@@ -133,6 +144,7 @@ program scft_driver
           maxval(resid(1:n_blk)), resid, (sum(R_blk(:, a)) / n, a = 1, n_blk), &
           (0d0, a = n_blk + 1, 3), maxval(abs(w_blk))
      flush(u)
+     if (logs) call logs_append(it, converged)
      done = it
      if (converged) exit
   end do
@@ -153,4 +165,63 @@ program scft_driver
      close(u)
   end if
   call MDE_release()
+
+contains
+
+  subroutine logs_create()
+    integer :: v
+    open(newunit=v, file='FreeEG.dat', status='replace')
+    close(v)
+    open(newunit=v, file='error.dat', status='replace')
+    close(v)
+    open(newunit=v, file='result.txt', status='replace')
+    close(v)
+  end subroutine logs_create
+
+  subroutine logs_append(iter, conv)
+    integer, intent(in) :: iter
+    logical, intent(in) :: conv
+    integer :: v, p, q
+    real(8) :: fh, r3(3)
+    fh = 0d0
+    do p = 1, n_blk
+       do q = p + 1, n_blk
+          fh = fh + chi(p, q) * f(p) * f(q)
+       end do
+    end do
+    r3 = 0d0
+    r3(1:n_blk) = resid(1:n_blk)
+    open(newunit=v, file='error.dat', status='old', position='append', recl=1024)
+    write(v, *) iter, r3
+    close(v)
+    open(newunit=v, file='FreeEG.dat', status='old', position='append', recl=1024)
+    if (conv) then
+       write(v, *) iter, FE_global, pff_global, tempEE_global, FE_global - fh
+       write(v, *) iter, box_x, 'FE-FH', FE_global - fh
+    else
+       write(v, *) iter, FE_global, pff_global, tempEE_global
+       write(v, *) iter, box_x, 'FE-FH', FE_global - fh
+       write(v, *) 'dx,dy', box_x / SIDEx, box_y / SIDEy
+       write(v, *) 'box size', box_x, box_y, box_z
+    end if
+    close(v)
+    if (conv) then
+       open(newunit=v, file='result.txt', status='old', position='append', recl=1024)
+       call result_lines(v, fh, conv)
+       close(v)
+       open(newunit=v, file='result.dat', status='replace', recl=1024)
+       call result_lines(v, fh, conv)
+       close(v)
+    end if
+  end subroutine logs_append
+
+  subroutine result_lines(v, fh, conv)
+    integer, intent(in) :: v
+    real(8), intent(in) :: fh
+    logical, intent(in) :: conv
+    write(v, *) 'FREE_ENERGY', FE_global, pff_global, tempEE_global, FE_global - fh
+    write(v, *) 'FREE_ENERGY-FH', FE_global - fh
+    write(v, *) 'box size', box_x, box_y, box_z
+    write(v, *) 'converge', conv
+  end subroutine result_lines
 end program scft_driver

@@ -128,6 +128,7 @@ module wormlike_MDE
   real(8) :: pff_global = 0d0                         ! ln Q   (SCFT_Density_FE.f90:91)
   real(8) :: totden = 0d0
   real(8) :: FE_global = 0d0                          ! F per chain (SCFT_Density_FE.f90:333)
+  real(8) :: tempEE_global = 0d0                      ! field-energy volume average (SCFT_Density_FE.f90:326-331)
 
   type(c_ptr), private, save :: handle = c_null_ptr
 
@@ -236,7 +237,8 @@ contains
        end do
        ar(r) = e - xi * (1d0 - sphi)
     end do
-    FE_global = -pff_global + periodic_simpson(ar) / (box_x * box_y * box_z)
+    tempEE_global = periodic_simpson(ar) / (box_x * box_y * box_z)
+    FE_global = -pff_global + tempEE_global
     deallocate(ar)
   end subroutine free_energy
 
