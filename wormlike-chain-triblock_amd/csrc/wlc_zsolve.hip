@@ -550,12 +550,15 @@ __global__ void __launch_bounds__(512) k_zsolve64t(Geom g, SolveArgs a) {
 // ---------------------------------------------------------------------------
 // nz = 128: two waves per (kx, ky) column
 // ---------------------------------------------------------------------------
-// Wave p of a column loads z = m + 64 p (lane m).  Forward: one radix-2 decimation-in-
-// frequency step across the two waves (through LDS), a = x(m) + x(m + 64),
-// b = (x(m) - x(m + 64)) w^m with w = exp(-2 pi i / 128), then the wave-local 64-point
-// transform, after which lane m of wave p holds kz = 2 m + p for all harmonics (the
-// solve needs nothing else).  Inverse: wave-local 64-point inverse, then
-// x(m) = a'(m) + w^-m b'(m), x(m + 64) = a'(m) - w^-m b'(m).  Workgroup = two columns.
+// Wave p of a column works on z = m + 64 p (lane m).  Forward: one radix-2 decimation-
+// in-frequency step, a = x(m) + x(m + 64) in wave 0, b = (x(m) - x(m + 64)) w^m in wave 1
+// with w = exp(-2 pi i / 128), then the wave-local 64-point transform, after which lane m
+// of wave p holds kz = 2 m + p for all harmonics (the solve needs nothing else).  Inverse:
+// wave-local 64-point inverse, then x(m) = a'(m) + w^-m b'(m), x(m + 64) = a'(m) - w^-m b'(m).
+// k_zsolve128 (workgroup = two adjacent columns of a tile) moves the columns between HBM and
+// LDS with coalesced accesses and forms the radix-2 step from LDS (z128_load_dif,
+// z128_dit_store); k_zsolve128s (one column per workgroup) loads each wave's half directly
+// and exchanges the halves through LDS (z128_exchange).
 constexpr int kXB = 16;  // harmonics per cross-wave exchange
 constexpr int kRegion128 = kRegion > kXB * 64 ? kRegion : kXB * 64;
 
@@ -597,6 +600,76 @@ __device__ __forceinline__ void z128_exchange(double2 (&v)[N], double2* own, con
   }
 }
 
+// The two columns of a k_zsolve128 workgroup are adjacent in the tile: element (h, z, cc)
+// of the pair at h * istride + 4 z + cc.  Thread t moves element (z = t >> 1, cc = t & 1)
+// of each harmonic (32 contiguous bytes per z for the pair), kPB harmonics per batch
+// through LDS rows of kPC entries (132: the two columns' rows fall on distinct banks).
+constexpr int kPB = 15;
+constexpr int kPC = 132;
+static_assert(kPB * 2 * kPC <= 4 * kRegion128, "pair batch fits the workgroup's LDS");
+
+// forward: every harmonic of the pair loaded at once (thread-owned elements land in v),
+// then per batch through LDS into the radix-2 decimation-in-frequency step: wave p of
+// column c gets x(m) + x(m + 64) (p = 0) or (x(m) - x(m + 64)) w^m (p = 1)
+template <class S>
+__device__ __forceinline__ void z128_load_dif(double2 (&v)[kM], double2* buf, const S* __restrict__ pair,
+                                              long long istride, int c, int p, int lane, double2 t,
+                                              bool real_col, bool io) {
+  const int tz = threadIdx.x >> 1, tc = threadIdx.x & 1;
+#pragma unroll
+  for (int h = 0; h < kM; ++h)
+    v[h] = io ? spec_ld(pair + (long long)h * istride + 4 * tz + tc) : make_double2(0.0, 0.0);
+#pragma unroll
+  for (int b0 = 0; b0 < kM; b0 += kPB) {
+    __syncthreads();  // the previous batch has been read
+#pragma unroll
+    for (int q = 0; q < kPB; ++q)
+      if (b0 + q < kM) buf[(q * 2 + tc) * kPC + tz] = v[b0 + q];
+    __syncthreads();
+    if (real_col) {
+#pragma unroll
+      for (int q = 0; q < kPB; ++q) {
+        if (b0 + q < kM) {
+          const double2 x0 = buf[(q * 2 + c) * kPC + lane], x1 = buf[(q * 2 + c) * kPC + lane + 64];
+          v[b0 + q] = p == 0 ? make_double2(x0.x + x1.x, x0.y + x1.y)
+                             : twiddle<-1>(make_double2(x0.x - x1.x, x0.y - x1.y), t);
+        }
+      }
+    }
+  }
+}
+
+// inverse: wave p of column c holds a'(m) (p = 0) or w^-m b'(m) (p = 1); per batch through
+// LDS, thread (z, cc) forms x(z) = a' + w^-m b' (z = m < 64) or a' - w^-m b' (z = m + 64)
+// and stores it
+template <class S>
+__device__ __forceinline__ void z128_dit_store(const double2 (&v)[kM], double2* buf, S* __restrict__ pair,
+                                               long long istride, int c, int p, int lane,
+                                               bool real_col, bool io) {
+  const int tz = threadIdx.x >> 1, tc = threadIdx.x & 1, m = tz & 63;
+#pragma unroll
+  for (int b0 = 0; b0 < kM; b0 += kPB) {
+    __syncthreads();  // the previous batch has been read
+    if (real_col) {
+#pragma unroll
+      for (int q = 0; q < kPB; ++q)
+        if (b0 + q < kM) buf[(q * 2 + c) * kPC + 64 * p + lane] = v[b0 + q];
+    }
+    __syncthreads();
+    if (io) {
+#pragma unroll
+      for (int q = 0; q < kPB; ++q) {
+        if (b0 + q < kM) {
+          const double2 a0 = buf[(q * 2 + tc) * kPC + m], b1 = buf[(q * 2 + tc) * kPC + 64 + m];
+          spec_st(pair + (long long)(b0 + q) * istride + 4 * tz + tc,
+                  tz < 64 ? make_double2(a0.x + b1.x, a0.y + b1.y)
+                          : make_double2(a0.x - b1.x, a0.y - b1.y));
+        }
+      }
+    }
+  }
+}
+
 template <int OCC, class S>
 __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   __shared__ double2 lds[4 * kRegion128];
@@ -618,12 +691,13 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   if (t4 * kKxTile + 2 * half >= nxh) return;  // both columns are padding (uniform)
   const int kxl = 2 * half + c;
   const int kx = t4 * kKxTile + kxl;
-  const bool real_col = kx < nxh;
+  const bool real_col = kx < nxh;                                     // wave-uniform
+  const bool io = t4 * kKxTile + 2 * half + (threadIdx.x & 1) < nxh;  // the column this thread moves
   const long long istride = spec_index(1, 0, 0, 0, ny, nt, 128);
-  // column kx: element (i, z) at i * istride + 4 z; this wave's half starts at z = 64 p
-  S* col = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 64 * p, ny, nt, 128) + kxl;
+  // the workgroup's column pair kx = 4 t4 + 2 half + {0, 1}: element (i, z, cc) at
+  // i * istride + 4 z + cc
+  S* pair = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, 128) + 2 * half;
   double2* wl = lds + wv * kRegion128;
-  const double2* partner = lds + (wv ^ 1) * kRegion128;
   double2 tw[8];
   {
     const int j = lane & 7;
@@ -632,12 +706,8 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   }
   const double2 t = g.az.tw[lane];  // exp(-2 pi i m / 128)
   double2 v[kM];
-  if (real_col) {
-#pragma unroll
-    for (int h = 0; h < kM; ++h) v[h] = spec_ld(col + (long long)h * istride + 4 * lane);
-  }
-  z128_exchange<-1, kM>(v, wl, partner, lane, p, t, real_col);
-  __syncthreads();  // the exchange regions are reused by the 64-point transforms
+  z128_load_dif(v, lds, pair, istride, c, p, lane, t, real_col, io);
+  __syncthreads();  // the batch rows are reused by the 64-point transforms
   if (real_col) z64_fwd_regs<kM>(v, wl, lane, tw);
   if (a.solve && real_col) {
     const double kxv = signed_k(kx, g.nx) * a.kfac[0];
@@ -652,11 +722,7 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
       for (int h = 0; h < kM; ++h) v[h] = twiddle<+1>(v[h], t);
     }
   }
-  z128_exchange<+1, kM>(v, wl, partner, lane, p, t, real_col);
-  if (real_col) {
-#pragma unroll
-    for (int h = 0; h < kM; ++h) spec_st(col + (long long)h * istride + 4 * lane, v[h]);
-  }
+  z128_dit_store(v, lds, pair, istride, c, p, lane, real_col, io);
 }
 
 // nz = 128 with the degree split of k_zsolve64s: workgroup = one column, four waves
