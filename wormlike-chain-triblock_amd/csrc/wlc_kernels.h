@@ -67,7 +67,8 @@ struct SolveArgs {
   int occ;             // nz = 64 kernel: target waves per SIMD (1 or 2)
   int split;           // Lmax 5-7, harmonics of a column split over two waves by degree:
                        // 0 off; nz = 64: 1 two columns per workgroup, 2 a whole 4-kx tile
-                       // per workgroup; nz = 128: 1 or 2 select the split kernel
+                       // per workgroup (FP64 spectrum; FP32 runs as 1); nz = 128: 1 or 2
+                       // select the split kernel
 };
 
 // Kernel C: inverse y and x (C2R) passes, scaling by 1/N, store of q(s).

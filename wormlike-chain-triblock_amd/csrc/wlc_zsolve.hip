@@ -9,6 +9,8 @@
 // kx columns of one harmonic is contiguous (4 nz complex values).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "wlc_device.h"
 
 #ifndef WLC_L
@@ -555,10 +557,11 @@ __global__ void __launch_bounds__(512) k_zsolve64t(Geom g, SolveArgs a) {
 // with w = exp(-2 pi i / 128), then the wave-local 64-point transform, after which lane m
 // of wave p holds kz = 2 m + p for all harmonics (the solve needs nothing else).  Inverse:
 // wave-local 64-point inverse, then x(m) = a'(m) + w^-m b'(m), x(m + 64) = a'(m) - w^-m b'(m).
-// k_zsolve128 (workgroup = two adjacent columns of a tile) moves the columns between HBM and
-// LDS with coalesced accesses and forms the radix-2 step from LDS (z128_load_dif,
-// z128_dit_store); k_zsolve128s (one column per workgroup) loads each wave's half directly
-// and exchanges the halves through LDS (z128_exchange).
+// k_zsolve128 (workgroup = two adjacent columns of a tile) with the FP64 spectrum moves the
+// columns between HBM and LDS with coalesced accesses and forms the radix-2 step from LDS
+// (z128_load_dif, z128_dit_store); with the FP32 spectrum, and in k_zsolve128s (one column
+// per workgroup), each wave loads its half directly and the halves are exchanged through
+// LDS (z128_exchange).
 constexpr int kXB = 16;  // harmonics per cross-wave exchange
 constexpr int kRegion128 = kRegion > kXB * 64 ? kRegion : kXB * 64;
 
@@ -706,8 +709,21 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   }
   const double2 t = g.az.tw[lane];  // exp(-2 pi i m / 128)
   double2 v[kM];
-  z128_load_dif(v, lds, pair, istride, c, p, lane, t, real_col, io);
-  __syncthreads();  // the batch rows are reused by the 64-point transforms
+  // FP64 spectrum: the column pair through LDS with coalesced accesses; FP32 spectrum
+  // (8-byte elements): direct loads of each wave's half and an exchange between the waves,
+  // which is faster there (128^3/L8: 1.9 vs 2.4 ms)
+  constexpr bool kPair = std::is_same<S, double2>::value;
+  [[maybe_unused]] S* col = pair + c + 64 * 4 * p;  // this wave's half column: (i, m) at i * istride + 4 m
+  if constexpr (kPair) {
+    z128_load_dif(v, lds, pair, istride, c, p, lane, t, real_col, io);
+  } else {
+    if (real_col) {
+#pragma unroll
+      for (int h = 0; h < kM; ++h) v[h] = spec_ld(col + (long long)h * istride + 4 * lane);
+    }
+    z128_exchange<-1, kM>(v, wl, lds + (wv ^ 1) * kRegion128, lane, p, t, real_col);
+  }
+  __syncthreads();  // the LDS rows are reused by the 64-point transforms
   if (real_col) z64_fwd_regs<kM>(v, wl, lane, tw);
   if (a.solve && real_col) {
     const double kxv = signed_k(kx, g.nx) * a.kfac[0];
@@ -722,7 +738,15 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
       for (int h = 0; h < kM; ++h) v[h] = twiddle<+1>(v[h], t);
     }
   }
-  z128_dit_store(v, lds, pair, istride, c, p, lane, real_col, io);
+  if constexpr (kPair) {
+    z128_dit_store(v, lds, pair, istride, c, p, lane, real_col, io);
+  } else {
+    z128_exchange<+1, kM>(v, wl, lds + (wv ^ 1) * kRegion128, lane, p, t, real_col);
+    if (real_col) {
+#pragma unroll
+      for (int h = 0; h < kM; ++h) spec_st(col + (long long)h * istride + 4 * lane, v[h]);
+    }
+  }
 }
 
 // nz = 128 with the degree split of k_zsolve64s: workgroup = one column, four waves
@@ -843,14 +867,17 @@ namespace {
 template <class S>
 hipError_t zsolve_launch(const Geom& g, const SolveArgs& a, hipStream_t st) {
   if (a.M != kM) return hipErrorInvalidValue;
-  if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split == 2) {
-    // one workgroup of eight waves per 4-kx tile
-    const size_t lds = sizeof(double2) * (size_t)kTileLds;
-    hipError_t e = hipFuncSetAttribute((const void*)k_zsolve64t<S>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_zsolve64t<S>), dim3(g.nt * g.ny), dim3(512), lds, st, g, a);
-    return hipGetLastError();
+  if constexpr (std::is_same<S, double2>::value) {
+    // one workgroup of eight waves per 4-kx tile; FP64 spectrum only: with the FP32 half
+    // spectrum (2 KB tiles) the two-column kernel below is faster (64^3/L6: 69 vs 93 us)
+    if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split == 2) {
+      const size_t lds = sizeof(double2) * (size_t)kTileLds;
+      hipError_t e = hipFuncSetAttribute((const void*)k_zsolve64t<S>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_zsolve64t<S>), dim3(g.nt * g.ny), dim3(512), lds, st, g, a);
+      return hipGetLastError();
+    }
   }
   if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
     // two workgroups per 4-kx tile
