@@ -188,15 +188,18 @@ def test_fused_inverse_rhs_is_bit_identical(wlc, case, monkeypatch):
     assert np.array_equal(phi0, phi1) and Q0 == Q1 and td0 == td1
 
 
-@pytest.mark.parametrize("shape", [(64, 6, 8), (128, 4, 8)])
-def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape):
+@pytest.mark.parametrize("shape,L", [((64, 6, 8), (5, 7, 6)), ((128, 4, 8), (5, 7, 6)),
+                                     ((24, 6, 12), (4, 5, 7)), ((32, 4, 6), (6, 4, 4))])
+def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     """Lmax 5..7 at nz = 64 and 128 runs the z solve with the harmonics of a column split
     over two waves (per z half at nz = 128); the unsplit kernel computes the same step
     (equal up to rounding: the split kernel passes k = 0 through identity-angle rotations
     instead of a diagonal shortcut).  nx/2 + 1 = 5 leaves padding columns in the last tile.
     At nz = 64 the whole-tile split kernel (the default) and the two-column split kernel
-    run the same operations (equal up to the compiler's multiply-add contraction)."""
-    box, Nblk, L, lam = (3.0, 2.0, 5.0), (3, 4, 3), (5, 7, 6), (1.2, 0.6, 1.0)
+    run the same operations (equal up to the compiler's multiply-add contraction).  At
+    other nz (24: the two parts share a wave; 32) the generic kernel splits Lmax 4..7 over
+    two threads per k-point."""
+    box, Nblk, lam = (3.0, 2.0, 5.0), (3, 4, 3), (1.2, 0.6, 1.0)
     rng = np.random.default_rng(31)
     w = 1.5 * rng.standard_normal((3,) + shape)
     out = []

@@ -859,6 +859,89 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
   }
 }
 
+// generic kernel B (nz other than 64 and 128), 4 <= Lmax <= 7, with the harmonics of each
+// k-point split by degree over two threads: thread t < TX nz holds l <= kLS of k-point
+// (tx, z), thread t + TX nz the same k-point's l > kLS, which doubles the threads of a
+// small grid and halves their registers.  The z transforms of a batch of eight harmonics
+// run over the 2 TX column-parts as one block-wide Stockham transform (element z of
+// column-part j of batch slot h at LDS (h 2 TX + j) nz + z, i.e. thread t's element at
+// h blockDim + t); the crossing chains are exchanged through LDS as in k_zsolve64s, one
+// region per 64 k-points.
+template <int N, int DIR>
+__device__ __forceinline__ void z_transform_n(double2 (&v)[N], double2* lds, const AxisDev& az,
+                                              int ncols, int lidx) {
+  const int nz = az.n;
+#pragma unroll
+  for (int b0 = 0; b0 < N; b0 += kHB) {
+    const int nb = (N - b0) < kHB ? (N - b0) : kHB;
+#pragma unroll
+    for (int h = 0; h < kHB; ++h)
+      if (b0 + h < N) lds[h * ncols * nz + lidx] = v[b0 + h];
+    __syncthreads();
+    fft_axis<DIR, kHB>(lds, az, nb * ncols, 1, nz);
+#pragma unroll
+    for (int h = 0; h < kHB; ++h)
+      if (b0 + h < N) v[b0 + h] = lds[h * ncols * nz + lidx];
+    __syncthreads();
+  }
+}
+
+template <class S>
+__global__ void __launch_bounds__(512) k_zsolve_split(Geom g, SolveArgs a) {
+  extern __shared__ double2 lds[];
+  const int nz = g.nz, ny = g.ny, nxh = g.nxh;
+  const int TX = a.TX;
+  const int np = TX * nz;  // threads per part; 2 TX nz == blockDim.x
+  const int t = threadIdx.x;
+  const int part = t >= np ? 1 : 0;
+  const int li = t - part * np;
+  const int tx = li / nz, z = li - tx * nz;
+  const int kx = blockIdx.x * TX + tx, ky = blockIdx.y;
+  const bool valid = kx < nxh;
+  const bool active = a.solve && valid;
+  const int nh = part ? PartHi::NH : PartLo::NH;
+  const int hb0 = part ? PartHi::B0 : PartLo::B0;
+  const long long istride = spec_index(1, 0, 0, 0, ny, g.nt, nz);
+  S* p = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, valid ? kx : 0, z, ny, g.nt, nz) +
+         (long long)hb0 * istride;
+  double2 v[kNV];
+#pragma unroll
+  for (int q = 0; q < kNV; ++q)
+    v[q] = (valid && q < nh) ? spec_ld(p + q * istride) : make_double2(0.0, 0.0);
+  z_transform_n<kNV, -1>(v, lds, g.az, 2 * TX, t);
+  // the transform's rows are free after its last barrier: crossing-chain regions
+  double2* xch = lds + (li >> 6) * kXch;
+  const int lane = li & 63;
+  const KFrame f(signed_k(kx, g.nx) * a.kfac[0], signed_k(ky, ny) * a.kfac[1],
+                 signed_k(z, nz) * a.kfac[2], a.sigma, a.ds);
+  if (active) {
+    if (part == 0) {
+      PartLo::to_frame(v, f);
+      tri_lower_forward<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+    } else {
+      PartHi::to_frame(v, f);
+    }
+  }
+  __syncthreads();
+  if (part == 1 && active) tri_upper<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+  __syncthreads();
+  if (active) {
+    if (part == 0) {
+      tri_lower_back<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+      PartLo::from_frame(v, f);
+    } else {
+      PartHi::from_frame(v, f);
+    }
+  }
+  __syncthreads();  // every crossing chain has been read before the transform reuses LDS
+  z_transform_n<kNV, +1>(v, lds, g.az, 2 * TX, t);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < kNV; ++q)
+      if (q < nh) spec_st(p + q * istride, v[q]);
+  }
+}
+
 }  // namespace per_l_<L>
 
 using namespace WLC_CAT(per_l_, WLC_L);
@@ -904,6 +987,20 @@ hipError_t zsolve_launch(const Geom& g, const SolveArgs& a, hipStream_t st) {
     return hipGetLastError();
   }
   if (a.TX * g.nz > 256 || a.TX < 1) return hipErrorInvalidValue;
+  if constexpr (kL >= 4 && kL <= 7) {
+    if (a.split) {
+      // two threads per k-point (degree split): 2 TX nz threads
+      const int np = a.TX * g.nz;
+      const size_t rows = (size_t)kHB * 2 * np, chains = (size_t)((np + 63) / 64) * kXch;
+      const size_t lds = sizeof(double2) * (rows > chains ? rows : chains);
+      hipError_t e = hipFuncSetAttribute((const void*)k_zsolve_split<S>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      dim3 grid((g.nxh + a.TX - 1) / a.TX, g.ny), block(2 * np);
+      hipLaunchKernelGGL(k_zsolve_split<S>, grid, block, lds, st, g, a);
+      return hipGetLastError();
+    }
+  }
   const size_t lds = sizeof(double2) * (size_t)kHB * a.TX * g.nz;
   hipError_t e = hipFuncSetAttribute((const void*)k_zsolve<S>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
