@@ -859,7 +859,8 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
   }
 }
 
-// generic kernel B (nz other than 64 and 128), 4 <= Lmax <= 7, with the harmonics of each
+// generic kernel B (nz other than 64 and 128, or any nz with the generic kernel forced),
+// 4 <= Lmax <= 7, with the harmonics of each
 // k-point split by degree over two threads: thread t < TX nz holds l <= kLS of k-point
 // (tx, z), thread t + TX nz the same k-point's l > kLS, which doubles the threads of a
 // small grid and halves their registers.  The z transforms of a batch of eight harmonics
