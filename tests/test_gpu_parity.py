@@ -63,6 +63,8 @@ CASES = [
     # 128 x 128 planes (128 KB of LDS per workgroup, radix 16 x 8 passes)
     ((4, 128, 128), (2.0, 6.0, 6.0), (3, 3), (2, 1), (0.5, 1.5), 0),
     ((2, 128, 128), (1.0, 4.0, 5.0), (2, 3, 2), (3, 2, 3), (1.0, 0.5, 2.0), 0),
+    # ... nz = 16 (the XCD z-range block order with two z-planes per XCD)
+    ((16, 128, 128), (2.0, 6.0, 6.0), (3, 3), (2, 1), (0.5, 1.5), 0),
 ]
 
 
