@@ -178,3 +178,12 @@ class Chain:
                 rho[b] += cw[j] * (q[s][:lim] * qs[s][:lim]).sum(0)
         totden = (Wv * rho.sum(0)).sum()
         return V * rho / totden, Q, totden
+
+
+def free_energy_ab(wA, wB, RA, RB, Q, chiN, box):
+    """Free energy per chain of the AB melt as the reference writes it
+    (SCFT_Density_FE.f90:296-298, :326-333): -ln Q plus the periodic-Simpson volume
+    average of  chiN RA RB - wA RA - wB RB + (wA + wB)/2 (RA + RB - 1)."""
+    ar = chiN * RA * RB - wA * RA - wB * RB + 0.5 * (wA + wB) * (RA + RB - 1.0)
+    Wv = volume_weights(wA.shape, box)
+    return -np.log(Q) + (Wv * ar).sum() / np.prod(box)

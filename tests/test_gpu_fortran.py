@@ -59,8 +59,11 @@ class Mixer:
         k = (it - self.ns) % (self.nh + 1)
         nr = min(it - self.ns, self.nh)
         self.wout[k], self.dw[k] = wo, wo - w
+        # denominator in the gauge <xi> = (1/n) sum_a sum_{b != a} chi_ab f_b
+        xbar = sum(chi[a][b] * f[b] for a in range(nb) for b in range(nb) if b != a) / nb
+        shift = xbar - self.vdot(xi, np.ones_like(xi))
         res = np.array([np.sqrt(abs(self.vdot(self.dw[k][a], self.dw[k][a]) /
-                                    self.vdot(wo[a], wo[a]))) for a in range(nb)])
+                                    self.vdot(wo[a] + shift, wo[a] + shift))) for a in range(nb)])
         slot = [(k - m) % (self.nh + 1) for m in range(nr + 1)]
         U = np.zeros((nr, nr))
         V = np.zeros(nr)

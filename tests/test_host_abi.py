@@ -46,6 +46,8 @@ def _params(wlc, **kw):
     dict(nx=7), dict(ny=10), dict(nz=0), dict(nx=512), dict(box=(2.0, -1.0, 2.0)), dict(nblk=4),
     dict(nblk=0), dict(Ns=13), dict(Nblk=(4, 0, 8)), dict(Lmax=(2, 9, 2)), dict(Lmax=(-1, 2, 2)),
     dict(lam=(1.0, float("nan"), 1.0)), dict(Ns=2, Nblk=(1, 1, 0), nblk=2),
+    # half planes beyond the 160 KB of LDS a plane kernel holds
+    dict(nx=144, ny=144), dict(nx=192, ny=128), dict(nx=256, ny=256),
 ])
 def test_create_rejects_bad_parameters(wlc, bad):
     lib = wlc.load()
@@ -55,6 +57,15 @@ def test_create_rejects_bad_parameters(wlc, bad):
     assert rc == -1  # WLC_EINVAL
     assert not h.value
     assert lib.wlc_mde_last_error()
+
+
+@pytest.mark.parametrize("nx,ny", [(144, 144), (192, 128), (256, 256)])
+def test_oversized_planes_name_the_limit(wlc, nx, ny):
+    lib = wlc.load()
+    p = _params(wlc, nx=nx, ny=ny)
+    h = ctypes.c_void_p()
+    assert lib.wlc_mde_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"LDS" in lib.wlc_mde_last_error()
 
 
 def test_reserved_fields_must_be_zero(wlc):

@@ -6,6 +6,7 @@ oracle is pinned by one of: numpy/scipy library routines, the independent numpy
 restatement in tests/golden (committed fixtures), closed forms, or invariants.
 """
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -281,3 +282,106 @@ def test_translation_invariance(oracle):
     phis, Qs, _ = orc.density(orc.forward(ws), orc.backward(ws))
     assert Qs == pytest.approx(Q, rel=1e-13)
     assert np.abs(np.roll(phi, (2, -2, 4), axis=(1, 2, 3)) - phis).max() < 1e-12
+
+
+# --- free energy (SCFT_Density_FE.f90:273-342; ABC form of the same terms) ----------------
+
+def _fe_ctx(oracle, shape, box, nblk):
+    Nblk = (4, 5, 3)[:nblk]
+    return oracle.Oracle(shape, box, Nblk, (1,) * nblk, (1.0,) * nblk)
+
+
+@pytest.mark.parametrize("nblk", [2, 3])
+def test_free_energy_homogeneous_closed_form(oracle, nblk):
+    """Uniform fields w_b and phi_b = f_b: F = -ln Q + sum_{a<b} chi_ab f_a f_b - sum_b w_b f_b
+    (the xi term vanishes because sum_b f_b = 1); w = 0, Q = 1 gives the Flory-Huggins
+    energy of the disordered melt exactly."""
+    shape, box = (4, 6, 8), (2.0, 3.0, 4.0)
+    orc = _fe_ctx(oracle, shape, box, nblk)
+    f = np.array([0.25, 0.45, 0.30][:nblk])
+    f = f / f.sum()
+    chi = np.zeros((3, 3))
+    chi[0, 1], chi[0, 2], chi[1, 2] = 13.0, 21.0, 17.5
+    phi = np.stack([np.full(shape, v) for v in f])
+    fh = sum(chi[a, b] * f[a] * f[b] for a in range(nblk) for b in range(a + 1, nblk))
+    assert orc.free_energy(np.zeros((nblk,) + shape), phi, 1.0, chi) == pytest.approx(fh, abs=1e-13)
+    wc = np.array([0.8, -1.7, 2.9][:nblk])
+    w = np.stack([np.full(shape, v) for v in wc])
+    Q = 0.37
+    ref = -np.log(Q) + fh - (wc * f).sum()
+    assert orc.free_energy(w, phi, Q, chi) == pytest.approx(ref, abs=1e-13)
+    orc.close()
+
+
+def test_free_energy_two_blocks_is_the_reference_form(oracle):
+    """Two blocks on random (w, phi, Q), phi not normalised: the oracle's ABC form
+    (pair sum, xi = mean field) equals the reference's AB expression
+    chiN RA RB - wA RA - wB RB + (wA + wB)/2 (RA + RB - 1), restated in numpy."""
+    sys.path.insert(0, GOLDEN)
+    import np_restatement as NR
+    shape, box = (6, 4, 8), (3.0, 2.0, 4.5)
+    orc = _fe_ctx(oracle, shape, box, 2)
+    rng = np.random.default_rng(21)
+    for _ in range(3):
+        w = 3.0 * rng.standard_normal((2,) + shape)
+        phi = 0.5 + 0.3 * rng.standard_normal((2,) + shape)
+        Q = float(rng.uniform(0.1, 5.0))
+        chiN = float(rng.uniform(5.0, 30.0))
+        chi = np.zeros((3, 3))
+        chi[0, 1] = chiN
+        ref = NR.free_energy_ab(w[0], w[1], phi[0], phi[1], Q, chiN, box)
+        assert orc.free_energy(w, phi, Q, chi) == pytest.approx(ref, rel=1e-13, abs=1e-13)
+    orc.close()
+
+
+def test_free_energy_gauge_invariance(oracle):
+    """A constant c added to every field multiplies Q by exp(-c) and leaves F unchanged,
+    for any phi (a wrong sign of the xi or w.phi term breaks this when sum_b phi_b != 1);
+    with Q held fixed F moves by exactly -c."""
+    shape, box = (4, 6, 8), (2.0, 3.0, 4.0)
+    orc = _fe_ctx(oracle, shape, box, 3)
+    rng = np.random.default_rng(22)
+    w = 2.0 * rng.standard_normal((3,) + shape)
+    phi = 0.33 + 0.2 * rng.standard_normal((3,) + shape)
+    chi = np.triu(rng.uniform(10, 30, (3, 3)), 1)
+    Q = 1.7
+    F = orc.free_energy(w, phi, Q, chi)
+    for c in (-3.0, 0.5, 7.25):
+        assert orc.free_energy(w + c, phi, Q * np.exp(-c), chi) == pytest.approx(F, abs=1e-12)
+        assert orc.free_energy(w + c, phi, Q, chi) == pytest.approx(F - c, abs=1e-12)
+    orc.close()
+
+
+def test_free_energy_pair_terms_are_additive(oracle):
+    """F is linear in chi with slope <phi_a phi_b> for each pair (a < b only: the lower
+    triangle of chi is not read)."""
+    shape, box = (4, 4, 6), (2.0, 2.0, 3.0)
+    orc = _fe_ctx(oracle, shape, box, 3)
+    rng = np.random.default_rng(23)
+    w = rng.standard_normal((3,) + shape)
+    phi = 0.33 + 0.2 * rng.standard_normal((3,) + shape)
+    F0 = orc.free_energy(w, phi, 1.0, np.zeros((3, 3)))
+    for a, b in [(0, 1), (0, 2), (1, 2)]:
+        chi = np.zeros((3, 3))
+        chi[a, b] = 4.0
+        slope = oracle.volume_integral(phi[a] * phi[b], box) / np.prod(box)
+        assert orc.free_energy(w, phi, 1.0, chi) - F0 == pytest.approx(4.0 * slope, rel=1e-12)
+        chi_lower = np.zeros((3, 3))
+        chi_lower[b, a] = 4.0
+        assert orc.free_energy(w, phi, 1.0, chi_lower) == pytest.approx(F0, abs=1e-13)
+    orc.close()
+
+
+def test_free_energy_of_disordered_sweep(oracle):
+    """End to end through the oracle's sweeps: w = 0 gives Q = 1 and phi = f, so F is the
+    disordered melt's sum_{a<b} chi_ab f_a f_b."""
+    shape, box, Nblk = (4, 4, 6), (2.0, 2.0, 3.0), (5, 7, 4)
+    orc = oracle.Oracle(shape, box, Nblk, (2, 3, 2), (1.0, 0.5, 2.0))
+    w = np.zeros((3,) + shape)
+    phi, Q, _ = orc.density(orc.forward(w), orc.backward(w))
+    chi = np.zeros((3, 3))
+    chi[0, 1], chi[0, 2], chi[1, 2] = 20.0, 15.0, 25.0
+    f = np.array(Nblk) / sum(Nblk)
+    fh = sum(chi[a, b] * f[a] * f[b] for a in range(3) for b in range(a + 1, 3))
+    assert orc.free_energy(w, phi, Q, chi) == pytest.approx(fh, abs=1e-12)
+    orc.close()

@@ -79,6 +79,12 @@ struct InvArgs {
   double* out;         // [M][nz][ny][nx]
 };
 
+// LDS of one workgroup on gfx950
+constexpr size_t kMaxLdsBytes = 160 * 1024;
+// true when the plane kernels can hold an nx x ny half plane (64 x 64 and 128 x 128 have
+// their own kernels; other sizes keep the whole ny (nx/2 + 1) half plane in LDS)
+bool plane_supported(int nx, int ny);
+
 hipError_t launch_rhs_fft(const Geom& g, const RhsArgs& a, int order, hipStream_t st);
 hipError_t launch_zsolve(const Geom& g, const SolveArgs& a, hipStream_t st);
 hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st);

@@ -606,6 +606,8 @@ int validate(const wlc_params* p) {
     if (!(p->box[a] > 0.0) || !std::isfinite(p->box[a])) return fail(WLC_EINVAL, "box edges must be positive");
   }
   if (p->nx > 256 || p->ny > 256 || p->nz > 256) return fail(WLC_EINVAL, "grid edges above 256 are not supported");
+  if (!wlc::plane_supported(p->nx, p->ny))
+    return fail(WLC_EINVAL, "the nx x ny half plane does not fit in LDS (ny (nx/2 + 1) must be <= 10240, or 64 x 64 / 128 x 128)");
   if (p->nblk < 1 || p->nblk > WLC_MAX_BLOCKS) return fail(WLC_EINVAL, "nblk must be 1..3");
   int tot = 0;
   for (int b = 0; b < p->nblk; ++b) {
@@ -903,6 +905,7 @@ int wlc_mde_set_box(wlc_handle* h, const double box[3]) {
     if (!(box[a] > 0.0) || !std::isfinite(box[a])) return fail(WLC_EINVAL, "box edges must be positive");
   for (int a = 0; a < 3; ++a) h->p.box[a] = box[a];
   h->fwd_done = h->bwd_done = false;
+  h->pre_recorded = false;
   return WLC_OK;
 }
 
@@ -913,6 +916,7 @@ int wlc_mde_set_w(wlc_handle* h, const double* w_host) {
   HIPCHK(hipStreamSynchronize(h->st));
   h->have_w = true;
   h->fwd_done = h->bwd_done = false;
+  h->pre_recorded = false;  // a concurrent backward sweep must start after the copy
   return WLC_OK;
 }
 
@@ -922,6 +926,7 @@ int wlc_mde_set_w_device(wlc_handle* h, const double* w_dev) {
   HIPCHK(copy_fields_in(h, h->w, w_dev, h->p.nblk, hipMemcpyDeviceToDevice));
   h->have_w = true;
   h->fwd_done = h->bwd_done = false;
+  h->pre_recorded = false;  // a concurrent backward sweep must start after the copy
   return WLC_OK;
 }
 

@@ -348,6 +348,15 @@ static size_t plane_lds_bytes(const Geom& g) {
   return sizeof(double2) * (specialised(g) ? n + n / 8 + 1 : n);
 }
 
+bool plane_supported(int nx, int ny) {
+  if ((nx == 64 && ny == 64) || (nx == 128 && ny == 128)) return true;
+  Geom g{};
+  g.nx = nx;
+  g.ny = ny;
+  g.nxh = nx / 2 + 1;
+  return plane_lds_bytes(g) <= kMaxLdsBytes && pick_maxe((long long)ny * g.nxh, kThreads) > 0;
+}
+
 }  // namespace wlc
 #include "wlc_plane64.h"
 #include "wlc_plane128.h"

@@ -1,7 +1,7 @@
 """Build the Fortran binding (module wormlike_MDE), the SCFT field update (module
 scft_iterate), the field files (module field_io) and the SCFT driver program with ROCm
-flang, linked against libwlc_mde.so (rpath to the package directory); and field_io_check,
-a GPU-free round trip of the field files."""
+flang, linked against libwlc_mde.so (rpath to the package directory); field_io_check,
+a GPU-free round trip of the field files; and fe_check, the GPU-free field energy."""
 import os
 import shutil
 import subprocess
@@ -12,6 +12,7 @@ PKG = os.path.dirname(HERE)
 BUILD = os.path.join(PKG, "_build", "fortran")
 EXE = os.path.join(HERE, "scft_driver")
 CHECK = os.path.join(HERE, "field_io_check")
+FE_CHECK = os.path.join(HERE, "fe_check")
 
 
 def find_flang():
@@ -30,22 +31,24 @@ def build():
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     steps = []
-    for src in ("wormlike_mde_gpu.f90", "scft_iterate.f90", "field_io.f90", "scft_driver.f90",
-                "field_io_check.f90"):
+    for src in ("scft_energy.f90", "wormlike_mde_gpu.f90", "scft_iterate.f90", "field_io.f90",
+                "scft_driver.f90", "field_io_check.f90", "fe_check.f90"):
         obj = os.path.join(BUILD, src.replace(".f90", ".o"))
         steps.append([fc, "-O2", "-module-dir", BUILD, "-I", BUILD, "-c", os.path.join(HERE, src),
                       "-o", obj])
         objs.append(obj)
-    tmp, tmp_check = EXE + ".tmp", CHECK + ".tmp"
-    steps.append([fc, "-o", tmp] + objs[:4] + [f"-L{PKG}", "-lwlc_mde", "-Wl,-rpath,$ORIGIN/.."])
-    steps.append([fc, "-o", tmp_check, objs[2], objs[4]])
+    tmp, tmp_check, tmp_fe = EXE + ".tmp", CHECK + ".tmp", FE_CHECK + ".tmp"
+    steps.append([fc, "-o", tmp] + objs[:5] + [f"-L{PKG}", "-lwlc_mde", "-Wl,-rpath,$ORIGIN/.."])
+    steps.append([fc, "-o", tmp_check, objs[3], objs[5]])
+    steps.append([fc, "-o", tmp_fe, objs[0], objs[6]])
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD)
         if r.returncode != 0:
             raise RuntimeError(f"Fortran build failed ({' '.join(cmd)}):\n{r.stderr[-4000:]}")
     os.replace(tmp, EXE)
     os.replace(tmp_check, CHECK)
-    print(f"built {EXE}, {CHECK}")
+    os.replace(tmp_fe, FE_CHECK)
+    print(f"built {EXE}, {CHECK}, {FE_CHECK}")
     return EXE
 
 

@@ -97,6 +97,7 @@ program scft_driver
         chi(a, b) = chi9(3 * (a - 1) + b)
      end do
   end do
+  chi_blk = chi
   if (sph_slot) then
      call split_operator_MDE_init()
   else
@@ -132,7 +133,7 @@ program scft_driver
         call MDE_qstar(.true.)
      end if
      call density()
-     call free_energy(chi)
+     call free_energy()
      call scft_update(it, chi, f, tol, converged)
      if (fp32_on .and. (converged .or. maxval(resid(1:n_blk)) < fp32_tol)) then
         ! the rest, including the convergence test, in FP64

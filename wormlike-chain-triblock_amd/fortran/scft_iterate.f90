@@ -11,9 +11,16 @@
 ! constant and leaves phi and F unchanged; its literal extension to three species
 ! (sum_{b /= a} chi_ab (phi_b - 1/2) + xi) has no fixed point with sum_b phi_b = 1.
 ! Residual per species: max_r |t_a| while mixing simply (:150-151), and
-! sqrt(<dW_a, dW_a> / <Wout_a, Wout_a>) in the Anderson phase (:206-215), where <a, b> is the
+! sqrt(<dW_a, dW_a> / <Wg_a, Wg_a>) in the Anderson phase (:206-215), where <a, b> is the
 ! periodic-Simpson volume average (dot_product_mpi, :1200-1237).  Converged when every
-! species' residual is below the tolerance (:1146).  The reference's extra exit after 150
+! species' residual is below the tolerance (:1146).
+! The reference divides by <Wout_a, Wout_a>, which depends on the field gauge: adding one
+! constant to every field changes nothing else (phi, F, t and the whole mixing trajectory)
+! but moves that denominator, and from zero-mean fields it is nearly zero.  Wg_a is Wout_a
+! in a fixed gauge: Wout_a - <xi> + xbar, with xbar = (1/n) sum_a sum_{b /= a} chi_ab f_b the
+! exchange field of the disordered melt, so the residual is a relative change of a field of
+! size O(chi N) whatever constant the fields carry (the reference's own initial fields,
+! init_w in init_array_2dcomp_mod.f90:843-1065, have means of that order).  The reference's extra exit after 150
 ! iterations with residuals below 5e-4 (:1151-1154) is not replicated.
 module scft_iterate
   use wormlike_MDE
@@ -96,8 +103,9 @@ contains
     integer, intent(in) :: iter
     real(8), intent(in) :: chi(3, 3), f(3), tol
     logical, intent(out) :: converged
-    real(8), allocatable :: t(:, :), wo(:, :), xi(:), wi(:, :), di(:, :), u(:, :), v(:)
+    real(8), allocatable :: t(:, :), wo(:, :), xi(:), wi(:, :), di(:, :), u(:, :), v(:), wg(:)
     integer :: n, a, b, k, m, q, nr, km, kq
+    real(8) :: xbar
     n = SIDEx * SIDEy * SIDEz
     allocate(t(0:n-1, n_blk), wo(0:n-1, n_blk), xi(0:n-1))
     xi = sum(w_blk(:, 1:n_blk), dim=2) / n_blk
@@ -123,9 +131,21 @@ contains
       nr = min(iter - n_simple, n_hist)
       w_out(:, :, k) = wo
       d_w(:, :, k) = wo - w_blk(:, 1:n_blk)
+      allocate(wg(0:n-1))
+      xbar = 0d0
       do a = 1, n_blk
-        resid(a) = sqrt(abs(vdot(d_w(:, a, k), d_w(:, a, k)) / vdot(w_out(:, a, k), w_out(:, a, k))))
+        do b = 1, n_blk
+          if (b /= a) xbar = xbar + chi(a, b) * f(b)
+        end do
       end do
+      xbar = xbar / n_blk
+      wg = 1d0
+      xbar = xbar - vdot(xi, wg)
+      do a = 1, n_blk
+        wg = w_out(:, a, k) + xbar
+        resid(a) = sqrt(abs(vdot(d_w(:, a, k), d_w(:, a, k)) / vdot(wg, wg)))
+      end do
+      deallocate(wg)
       allocate(u(nr, nr), v(nr), wi(0:n-1, n_blk), di(0:n-1, n_blk))
       u = 0d0
       v = 0d0

@@ -112,6 +112,7 @@ end module wlc_mde_c
 module wormlike_MDE
   use iso_c_binding
   use wlc_mde_c
+  use scft_energy
   implicit none
 
   ! ---- globals the driver sets (the reference keeps these in global_para) ----
@@ -123,6 +124,7 @@ module wormlike_MDE
   integer :: L_bar_blk(3) = 0                         ! Lmax per block
   real(8) :: lambda_blk(3) = 1d0                      ! Loa/(2 kappa) per block
   real(8), allocatable :: w_blk(:,:)                  ! (0:LOCAL_SIZE-1, 1:n_blk), x fastest
+  real(8) :: chi_blk(3,3) = 0d0                       ! chi_ab N (upper triangle read; AB: NXab)
   ! ---- outputs (density / free_energy) ----
   real(8), allocatable :: R_blk(:,:)                  ! phi per block, same layout as w_blk
   real(8) :: pff_global = 0d0                         ! ln Q   (SCFT_Density_FE.f90:91)
@@ -217,49 +219,25 @@ contains
   end subroutine density
 
   ! F = -ln Q + (1/V) int [ sum_{a<b} chi_ab phi_a phi_b - sum_a w_a phi_a - xi (1 - sum_a phi_a) ],
-  ! xi = mean of the block fields; for two blocks this is SCFT_Density_FE.f90:296-298.
+  ! xi = mean of the block fields; for two blocks this is SCFT_Density_FE.f90:296-298, :333.
+  ! Like the reference's free_energy() it reads the module globals (w_blk, R_blk, pff_global
+  ! and the interaction matrix chi_blk, the reference's NXab); a driver may pass chi instead.
   subroutine free_energy(chi)
-    real(8), intent(in) :: chi(3, 3)
-    real(8), allocatable :: ar(:)
-    integer :: r, a, b, n
-    real(8) :: xi, sphi, e
-    n = SIDEx * SIDEy * SIDEz
-    allocate(ar(0:n-1))
-    do r = 0, n - 1
-       xi = sum(w_blk(r, 1:n_blk)) / n_blk
-       sphi = sum(R_blk(r, 1:n_blk))
-       e = 0d0
-       do a = 1, n_blk
-          do b = a + 1, n_blk
-             e = e + chi(a, b) * R_blk(r, a) * R_blk(r, b)
-          end do
-          e = e - w_blk(r, a) * R_blk(r, a)
-       end do
-       ar(r) = e - xi * (1d0 - sphi)
-    end do
-    tempEE_global = periodic_simpson(ar) / (box_x * box_y * box_z)
+    real(8), intent(in), optional :: chi(3, 3)
+    if (present(chi)) then
+       tempEE_global = field_energy(w_blk, R_blk, chi, n_blk, SIDEx, SIDEy, SIDEz, &
+            [box_x, box_y, box_z])
+    else
+       tempEE_global = field_energy(w_blk, R_blk, chi_blk, n_blk, SIDEx, SIDEy, SIDEz, &
+            [box_x, box_y, box_z])
+    end if
     FE_global = -pff_global + tempEE_global
-    deallocate(ar)
   end subroutine free_energy
 
-  ! periodic Simpson volume integral, weights 2/3 (even index) and 4/3 (odd index)
-  ! per axis (utility_mod.f90:238-298)
+  ! periodic Simpson volume integral over the module grid (utility_mod.f90:238-298)
   real(8) function periodic_simpson(f) result(s)
     real(8), intent(in) :: f(0:)
-    integer :: x, y, z
-    real(8) :: cx, cy, cz
-    s = 0d0
-    do z = 0, SIDEz - 1
-       cz = merge(4d0/3d0, 2d0/3d0, mod(z, 2) == 1)
-       do y = 0, SIDEy - 1
-          cy = merge(4d0/3d0, 2d0/3d0, mod(y, 2) == 1)
-          do x = 0, SIDEx - 1
-             cx = merge(4d0/3d0, 2d0/3d0, mod(x, 2) == 1)
-             s = s + cx * cy * cz * f(x + SIDEx * (y + SIDEy * z))
-          end do
-       end do
-    end do
-    s = s * (box_x / SIDEx) * (box_y / SIDEy) * (box_z / SIDEz)
+    s = periodic_simpson_grid(f, SIDEx, SIDEy, SIDEz, [box_x, box_y, box_z])
   end function periodic_simpson
 
   subroutine MDE_release()
