@@ -30,18 +30,8 @@ KERNELS = ["rhs_fft_xy", "zsolve", "ifft_xy", "density_acc", "inv_rhs_fft_xy"]
 def workload_w(shape=(64, 64, 64), seed=1, sigma=2.0, std=5.0, nblk=3):
     """Seeded block fields: N(0,1), periodic Gaussian filter exp(-sigma^2 |2 pi m/n|^2 / 2)
     in cells, then each block scaled to the given standard deviation."""
-    rng = np.random.default_rng(seed)
-    w = rng.standard_normal((nblk,) + tuple(shape))
-    nz, ny, nx = shape
-    kz = 2 * np.pi * np.fft.fftfreq(nz)[:, None, None]
-    ky = 2 * np.pi * np.fft.fftfreq(ny)[None, :, None]
-    kx = 2 * np.pi * np.fft.rfftfreq(nx)[None, None, :]
-    filt = np.exp(-0.5 * sigma ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
-    for b in range(nblk):
-        f = np.fft.irfftn(np.fft.rfftn(w[b]) * filt, s=shape, axes=(0, 1, 2))
-        f -= f.mean()
-        w[b] = f * (std / f.std())
-    return w
+    import workloads
+    return workloads.smoothed_fields(shape, seed, sigma, std, nblk)
 
 
 def algorithmic_bytes(M, N, Nk):
@@ -103,11 +93,8 @@ def cpu_baseline(w, target_s=15.0):
 def workload_w128(n=128, seed=2):
     """w_b = s_b (10 (sin X cos Y + sin Y cos Z + sin Z cos X) + 0.5 N(0,1)), X = 2 pi x / box,
     one N(0,1) field from the seed, block signs s = (+1, 0, -1)."""
-    x = np.arange(n) * (2 * np.pi / n)
-    X, Y, Z = x[None, None, :], x[None, :, None], x[:, None, None]
-    base = 10.0 * (np.sin(X) * np.cos(Y) + np.sin(Y) * np.cos(Z) + np.sin(Z) * np.cos(X))
-    f = base + 0.5 * np.random.default_rng(seed).standard_normal((n, n, n))
-    return np.stack([s * f for s in (1.0, 0.0, -1.0)])
+    import workloads
+    return workloads.gyroid_fields(n, seed)
 
 
 def bench_grid128(W, args):
@@ -250,6 +237,80 @@ def bench_spectrum_fp32(W, args):
     }))
 
 
+def c4_cases(n):
+    import batch_case as B
+    return B.draw()[:n]
+
+
+def c4_report(kind, rows, extra):
+    """JSON line of a time-to-residual run over the batching workload's instances."""
+    t = np.array([r["time_s"] for r in rows])
+    it = np.array([r["iterations"] for r in rows])
+    conv = [r["converged"] for r in rows]
+    tc = t[np.array(conv, dtype=bool)] if any(conv) else np.array([np.nan])
+    out = {
+        "metric": f"time-to-residual 1e-6, {kind}",
+        "value": round(float(np.mean(t)), 4), "unit": "s", "higher_is_better": False,
+        "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+        "instances": len(rows), "converged": int(sum(conv)),
+        "mean_s": round(float(np.mean(t)), 4), "median_s": round(float(np.median(t)), 4),
+        "mean_s_converged": round(float(np.mean(tc)), 4),
+        "mean_iterations": round(float(np.mean(it)), 1), "median_iterations": float(np.median(it)),
+        "config": {"workload": "48 linear ABC wormlike triblocks from seed 3: cubic grid edge "
+                               "{24,32,48,64}, Lmax {4,6}, f_A, f_B ~ U[0.2,0.45], chiN_ij ~ U[12,30], "
+                               "lambda ~ U[0.25,4], box ~ U[3,6], Ns 100, w = 0.1 N(0,1); 20 "
+                               "simple-mixing iterations (0.05) then Anderson (10 histories, 0.1), "
+                               "at most 520 iterations (500 Anderson), residual 1e-6",
+                   "mixing": "simple then Anderson"},
+        "per_instance": [{k: (round(v, 10) if isinstance(v, float) else v) for k, v in r.items()}
+                         for r in rows],
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_plain_c4(args):
+    """Each instance alone through the Fortran driver (module wormlike_MDE on one handle,
+    field update on the host), one after the other; time = the driver's loop wall time."""
+    import scft_case as S
+    rows = []
+    for i, c in enumerate(c4_cases(args.c4)):
+        r, info = S.run_case(method=1, max_iter=520, n_simple=20, n_hist=10, lam_s=0.05, lam_a=0.1,
+                             tol=1e-6, case=c, timeout=900)
+        rows.append({"i": i, "edge": c["shape"][0], "L": c["L"][0], "iterations": info["iterations"],
+                     "converged": bool(info["converged"]), "time_s": info["seconds"],
+                     "F": float(r[-1, 1]) if len(r) else None, "error": info["error"]})
+        print(f"# plain {i}: {rows[-1]}", file=sys.stderr, flush=True)
+    c4_report("plain single-instance SCFT (Fortran driver, one instance at a time)", rows, {})
+
+
+def bench_batch_c4(args, W):
+    """All instances in one batch (wlc_batch: one launch per kernel variant and contour step
+    for every instance, field update on the device, one SCFT iteration replayed from a HIP
+    graph); time of an instance = wall time from the first iteration to the end of the
+    iteration in which it converged (or stopped)."""
+    cases = c4_cases(args.c4)
+    b = W.Batch(cases, W.scft_opts(max_iter=520, n_simple=20, n_hist=10, lam_simple=0.05,
+                                   lam_anderson=0.1, tol=1e-6))
+    for i, c in enumerate(cases):
+        b.set_w(i, c["w"])
+    t0 = time.perf_counter()
+    while b.iterate(20, use_graph=True) > 0:
+        pass
+    wall = time.perf_counter() - t0
+    rows = []
+    for i, c in enumerate(cases):
+        st = b.status(i)
+        rows.append({"i": i, "edge": c["shape"][0], "L": c["L"][0], "iterations": st["iterations"],
+                     "converged": st["converged"], "time_s": st["time_ms"] / 1e3, "F": st["F"],
+                     "resid": max(st["resid"])})
+    launches = b.launches(True)
+    b.close()
+    c4_report("batched SCFT (all instances per launch, device field update, HIP graph)", rows,
+              {"batch_wall_s": round(wall, 3), "amortised_s_per_instance": round(wall / len(rows), 4),
+               "launches_per_iteration": launches})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +333,12 @@ def main():
                     help="the 64^3 sweep with the half spectrum stored as FP32 (all arithmetic "
                          "FP64), one JSON line of its own with the phi / Q / F differences "
                          "from an FP64 run on the same fields")
+    ap.add_argument("--plain-c4", action="store_true",
+                    help="time-to-residual of the batching workload's instances run one at a "
+                         "time through the Fortran driver (the plain comparator)")
+    ap.add_argument("--batch-c4", action="store_true",
+                    help="time-to-residual of the same instances advanced together in one batch")
+    ap.add_argument("--c4", type=int, default=48, help="instances of the batching workload")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the counter "
                          "calibration kernel of wormlike-chain-triblock_amd/profile_summary.py "
@@ -323,8 +390,15 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 3}))
         return
 
+    if args.plain_c4:
+        bench_plain_c4(args)
+        return
+
     import wlc_mde as W
 
+    if args.batch_c4:
+        bench_batch_c4(args, W)
+        return
     if args.grid128:
         bench_grid128(W, args)
         return

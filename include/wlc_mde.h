@@ -144,6 +144,115 @@ int wlc_mde_get_timing(wlc_handle* h, double* ms, long long* count, int nkernels
 int wlc_mde_event_record(wlc_handle* h, int slot);
 int wlc_mde_event_elapsed(wlc_handle* h, int slot_a, int slot_b, double* ms);
 
+/* One contour step of the handle's chain on caller device buffers: the step of direction
+ * dir (+1 forward, -1 backward) whose target is contour point s, i.e. the BDF right-hand
+ * side from hist_dev[j] = slice s - dir (j + 1) (j < BDF order = min(s, 3) forward,
+ * min(Ns - s, 3) backward; entries beyond the order are not read), the R2C FFT, the solve
+ * of G(k) q = P and the C2R FFT into out_dev.  Slices are dense [M][nz][ny][nx] with the M
+ * of the block owning their contour point, w_dev is [nblk][nz][ny][nx]; the target takes
+ * its block's field, Lmax and lambda and the history is zero-padded / truncated to its
+ * harmonics, as in the sweeps.  Queued on the handle's stream (no host wait); the handle's
+ * half spectrum is the scratch.  Replaces the body of one iteration of the contour loops of
+ * MDE_q / MDE_qstar (wormlike_MDE_2dcomp_mod.f90:323-455, :584-650; :1035-1110). */
+int wlc_mde_step(wlc_handle* h, int dir, int s, const double* const hist_dev[3],
+                 const double* w_dev, double* out_dev);
+
+/* ---- Batched SCFT: many independent instances on one GPU -------------------------------
+ * The reference runs one SCFT per process (cal_scft, SCFT_control.f90:235-262: MDE_q,
+ * MDE_qstar, density, free_energy, iterate; field update AB_iterate, SCFT_iterate.f90:27-333).
+ * A batch holds n instances, each with its own wlc_params (grid, box, blocks, Lmax, lambda;
+ * keep_qstar = ckpt_every = 0: the full forward store must fit) and interaction matrix
+ * chi[i][3][3] (chi_ab N, row a, column b), and advances all of them together: each contour
+ * step is one launch per kernel variant covering every instance, and the field update
+ * (simple, then Anderson mixing; FP64) runs on the device.  One SCFT iteration of all
+ * instances is a fixed launch sequence, replayed from a HIP graph when use_graph is set.
+ * The field update is that of the Fortran host module scft_iterate (forces, residuals,
+ * convergence test; Anderson coefficients from a Gram matrix of the dW history). */
+typedef struct wlc_batch wlc_batch;
+
+typedef struct wlc_scft_opts {
+  int max_iter;         /* iterations per instance (simple + Anderson) */
+  int n_simple;         /* simple-mixing iterations before Anderson (Num_simple_mixing_AB) */
+  int n_hist;           /* Anderson history depth, 1..16 (Anderson_nim_AB) */
+  int form;             /* 0: w = sum c Wout + lam sum c dW (the reference's update);
+                           1: w = sum c w + lam sum c dW */
+  double lam_simple;    /* simple-mixing step (lanbtWA / lanbtWB) */
+  double lam_anderson;  /* Anderson step (lambda_WAB_anderson) */
+  double tol;           /* converged when every species' residual is below tol */
+  int reserved[4];      /* must be zero */
+} wlc_scft_opts;
+
+typedef struct wlc_scft_status {
+  int iterations;       /* iterations done */
+  int converged;        /* 1: residual below tol */
+  int done;             /* converged or max_iter reached: the instance no longer advances */
+  int pad;
+  double F;             /* free energy per chain of the last iteration (kT) */
+  double lnQ;
+  double resid[3];      /* residual per species of the last iteration */
+  double time_ms;       /* wall time from the first iteration to the end of the iteration in
+                           which the instance finished (-1 while it runs) */
+  double elapsed_ms;    /* wall time of all iterations so far */
+} wlc_scft_status;
+
+/* Create a batch of n instances (params[n], chi[n][9]); opts NULL = max_iter 500,
+ * n_simple 20, n_hist 10, form 0, lam_simple 0.05, lam_anderson 0.1, tol 1e-6.
+ * Replaces starting n reference processes (SCFT_main.f90:1-55). */
+int wlc_batch_create(const wlc_params* params, const double* chi, int n, const wlc_scft_opts* opts,
+                     wlc_batch** out);
+/* number of instances */
+int wlc_batch_size(const wlc_batch* b);
+/* launches of one sweep pair with observables (with_update = 0) or of one SCFT iteration */
+int wlc_batch_launches(const wlc_batch* b, int with_update);
+/* fields of instance i, [nblk][nz][ny][nx] host arrays (the module globals wA, wB / w_blk) */
+int wlc_batch_set_w(wlc_batch* b, int i, const double* w_host);
+int wlc_batch_get_w(wlc_batch* b, int i, double* w_host);
+/* forward and backward sweeps, Q, totden and phi of every instance still advancing
+ * (MDE_q, MDE_qstar and density of each, wormlike_MDE_2dcomp_mod.f90:231-1557,
+ * SCFT_Density_FE.f90:3-159) */
+int wlc_batch_sweep(wlc_batch* b);
+/* Q, totden and phi [nblk][nz][ny][nx] of instance i after wlc_batch_sweep or an iteration */
+int wlc_batch_density(wlc_batch* b, int i, double* phi_host, double* Q, double* totden);
+/* forward slice s of instance i, [M(s)][nz][ny][nx] (tests) */
+int wlc_batch_get_q(wlc_batch* b, int i, int s, double* q_host);
+/* up to niter SCFT iterations of every instance still advancing (sweeps, density, free
+ * energy, field update, convergence test); stops early when none is left; *n_active =
+ * instances still advancing */
+int wlc_batch_iterate(wlc_batch* b, int niter, int use_graph, int* n_active);
+int wlc_batch_status(wlc_batch* b, int i, wlc_scft_status* st);
+void wlc_batch_destroy(wlc_batch* b);
+
+/* ---- Warm-start network ------------------------------------------------------------------
+ * A 3D convolutional U-Net (levels n, n/2, n/4, n/8; base 32 channels; 1.9 M parameters)
+ * that proposes initial SCFT fields.  The reference has no counterpart: every SCFT starts
+ * from the hand-built fields of init_w (init_array_2dcomp_mod.f90:843-1065), which this
+ * replaces as the source of w at iteration 0; plain SCFT still decides the converged state.
+ * Inputs x are [batch][n][n][n][11] (3 seed fields, then f_A, f_B, f_C, chi_AB, chi_BC,
+ * chi_AC, lambda, box broadcast), outputs and targets [batch][n][n][n][3]; FP32 host arrays.
+ * Training and inference run hand-written kernels: bf16 matrix-core convolutions with FP32
+ * accumulation, FP32 master weights and Adam. */
+typedef struct wlc_unet wlc_unet;
+int wlc_unet_create(int n, int batch, unsigned long long seed, wlc_unet** out);
+/* trainable parameters (padded = 1: the flat layout's length, including zero padding) */
+long long wlc_unet_num_params(const wlc_unet* u, int padded);
+/* layer i: info[8] = cin, cout (padded), cin, cout (real), level, relu, weight offset, bias
+ * offset in the flat parameters (W [cout][27][cin], b [cout]) */
+int wlc_unet_layer(const wlc_unet* u, int i, int* info);
+int wlc_unet_forward(wlc_unet* u, const float* x, float* y);
+/* forward + backward: parameter gradients of the mean-squared error against t; *loss */
+int wlc_unet_gradients(wlc_unet* u, const float* x, const float* t, float* loss);
+/* one Adam step (beta 0.9 / 0.999) with the last gradients */
+int wlc_unet_adam(wlc_unet* u, float lr);
+int wlc_unet_train_step(wlc_unet* u, const float* x, const float* t, float lr, float* loss);
+/* flat parameters (which = 0) or gradients (which = 1) to host; parameters from host */
+int wlc_unet_get(wlc_unet* u, int which, float* out);
+int wlc_unet_set_params(wlc_unet* u, const float* in);
+void wlc_unet_destroy(wlc_unet* u);
+/* Test hook: one convolution (bf16 operands, FP32 accumulation) and its gradients on host
+ * arrays: x [V][cin], w [cout][27][cin], b [cout], dy [V][cout] -> y, dx, dw, db. */
+int wlc_test_conv3d(int B, int D, int H, int W, int cin, int cout, const float* x, const float* w,
+                    const float* b, const float* dy, float* y, float* dx, float* dw, float* db);
+
 void wlc_mde_destroy(wlc_handle* h);
 
 /* message of the last failure on this thread ("" if none) */

@@ -28,9 +28,12 @@ FLAGS = [
 def _units():
     units = [
         ("wlc_mde.hip", "wlc_mde.o", []),
+        ("wlc_batch.hip", "wlc_batch.o", []),
+        ("wlc_unet.hip", "wlc_unet.o", []),
         ("wlc_plane.hip", "wlc_plane.o", []),
         ("wlc_dispatch.cpp", "wlc_dispatch.o", ["-x", "hip"]),
         ("sh_setup.cpp", "sh_setup.o", []),
+        ("wlc_tune.cpp", "wlc_tune.o", []),
     ]
     for L in range(MAX_L + 1):
         units.append(("wlc_zsolve.hip", f"wlc_zsolve_L{L}.o", [f"-DWLC_L={L}"]))

@@ -8,6 +8,8 @@ namespace wlc {
 
 #define WLC_DECL(L)                                                                          \
   hipError_t zsolve_launch_L##L(const Geom&, const SolveArgs&, hipStream_t);                 \
+  hipError_t zsolve_batch_L##L(const ZJob&, const ZJob*, int, hipStream_t);                 \
+  LaunchKey zsolve_key_L##L(const Geom&, const SolveArgs&);                                  \
   hipError_t test_solve_L##L(int, const double*, double, double, double, double, double2*,   \
                              hipStream_t);                                                   \
 
@@ -22,6 +24,26 @@ hipError_t launch_zsolve(const Geom& g, const SolveArgs& a, hipStream_t st) {
 #undef WLC_CASE
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_zsolve_batch(const ZJob& rep, const ZJob* jobs, int n, hipStream_t st) {
+  switch (rep.a.L) {
+#define WLC_CASE(L) \
+  case L: return zsolve_batch_L##L(rep, jobs, n, st);
+    WLC_CASE(0) WLC_CASE(1) WLC_CASE(2) WLC_CASE(3) WLC_CASE(4) WLC_CASE(5) WLC_CASE(6) WLC_CASE(7) WLC_CASE(8)
+#undef WLC_CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+LaunchKey zsolve_key(const Geom& g, const SolveArgs& a) {
+  switch (a.L) {
+#define WLC_CASE(L) \
+  case L: return zsolve_key_L##L(g, a);
+    WLC_CASE(0) WLC_CASE(1) WLC_CASE(2) WLC_CASE(3) WLC_CASE(4) WLC_CASE(5) WLC_CASE(6) WLC_CASE(7) WLC_CASE(8)
+#undef WLC_CASE
+  }
+  return LaunchKey{nullptr, 0, 0, 0, 0};
 }
 
 hipError_t launch_test_solve(int L, int nk, const double* kvec, double cdelta, double sigma,

@@ -79,6 +79,22 @@ struct InvArgs {
   double* out;         // [M][nz][ny][nx]
 };
 
+// One instance of a batched launch: the arguments a single launch of the same kernel would
+// take.  A batched launch runs the kernel's grid once per job (job = blockIdx.z); every job
+// of one launch selects the same kernel (same grid sizes, Lmax, spectrum type and BDF
+// order).  A job whose *skip is nonzero returns at once (an instance that has converged).
+struct PlaneJob {
+  Geom g;
+  InvArgs c;         // kernel C part (inverse xy, store of q(s))
+  RhsArgs a;         // kernel A part (right-hand side, forward xy)
+  const int* skip;   // nullptr: always run
+};
+struct ZJob {
+  Geom g;
+  SolveArgs a;
+  const int* skip;
+};
+
 // LDS of one workgroup on gfx950
 constexpr size_t kMaxLdsBytes = 160 * 1024;
 // true when the plane kernels can hold an nx x ny half plane (64 x 64 and 128 x 128 have
@@ -93,6 +109,27 @@ hipError_t launch_ifft_store(const Geom& g, const InvArgs& a, hipStream_t st);
 // and writes the next step's half spectrum for a.M harmonics
 hipError_t launch_inv_rhs(const Geom& g, const InvArgs& c, const RhsArgs& a, int order,
                           hipStream_t st);
+
+// batched twins: `jobs` is a device array of n jobs whose kernel selection equals that of
+// the representative rep (the host groups jobs so); same stream semantics
+hipError_t launch_rhs_fft_batch(const PlaneJob& rep, int order, const PlaneJob* jobs, int n,
+                                hipStream_t st);
+hipError_t launch_ifft_store_batch(const PlaneJob& rep, const PlaneJob* jobs, int n, hipStream_t st);
+hipError_t launch_inv_rhs_batch(const PlaneJob& rep, int order, const PlaneJob* jobs, int n,
+                                hipStream_t st);
+hipError_t launch_zsolve_batch(const ZJob& rep, const ZJob* jobs, int n, hipStream_t st);
+// key of the kernel (and grid) a plane / z-solve launch with these arguments selects: jobs
+// with equal keys may share a batched launch
+struct LaunchKey {
+  const void* k;
+  unsigned gx, gy, threads;
+  size_t lds;
+  bool operator==(const LaunchKey& o) const {
+    return k == o.k && gx == o.gx && gy == o.gy && threads == o.threads && lds == o.lds;
+  }
+};
+LaunchKey plane_key(const Geom& g, int kind, int order, int planes);
+LaunchKey zsolve_key(const Geom& g, const SolveArgs& a);
 
 // small elementwise/reduction kernels
 hipError_t launch_set_delta(const Geom& g, double* slice, int M, hipStream_t st);

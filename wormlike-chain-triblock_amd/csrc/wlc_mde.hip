@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/wlc_mde.h"
+#include "wlc_host.h"
 #include "wlc_internal.h"
 #include "wlc_kernels.h"
 
@@ -75,7 +76,7 @@ struct wlc_handle {
   long long zs = 0, hs = 0;  // padded real-space strides
   int nxh = 0, Ns = 0, Mmax = 0;
   double ds = 0.0;
-  int Mb[3] = {0, 0, 0}, Lb[3] = {0, 0, 0}, start[3] = {0, 0, 0}, TX[3] = {1, 1, 1};
+  int Mb[3] = {0, 0, 0}, Lb[3] = {0, 0, 0}, start[3] = {0, 0, 0};
   std::vector<long long> off;  // forward store slice offsets (doubles)
   std::vector<std::vector<double>> cw;  // contour weights per block
   double* q = nullptr;
@@ -138,9 +139,6 @@ struct wlc_handle {
   bool seg_busy[2] = {false, false};
   int dens_seg = -1;  // segment whose densities are being queued
   bool have_w = false, fwd_done = false, bwd_done = false;
-  int generic_zsolve = 0;
-  int zs_occ = 1;
-  int zs_split = 2;  // Lmax 5-7 harmonics split by degree: 1 two-column, 2 whole-tile (nz = 64)
   int fuse = 1;  // kernel C of a step fused with kernel A of the next
   // timing
   bool timing = false;
@@ -276,6 +274,25 @@ SweepCtx main_sweep(const wlc_handle* h, int dir) {
   return c;
 }
 
+// BDF1/2/3 coefficients: P = sum_j a_j q(s - dir (j + 1)) - ds w sum_j b_j q(...), and the
+// diagonal c_delta of G (wormlike_MDE_2dcomp_mod.f90:341, :499-500, :604-606;
+// Gmatrix_2dcomp_mod.f90:384-429)
+void bdf_coeffs(int order, double a[3], double b[3], double* cdelta) {
+  if (order == 1) {
+    a[0] = 1.0; a[1] = 0.0; a[2] = 0.0;
+    b[0] = 1.0; b[1] = 0.0; b[2] = 0.0;
+    *cdelta = 1.0;
+  } else if (order == 2) {
+    a[0] = 2.0; a[1] = -0.5; a[2] = 0.0;
+    b[0] = 2.0; b[1] = -1.0; b[2] = 0.0;
+    *cdelta = 1.5;
+  } else {
+    a[0] = 3.0; a[1] = -1.5; a[2] = 1.0 / 3.0;
+    b[0] = 3.0; b[1] = -3.0; b[2] = 1.0;
+    *cdelta = 11.0 / 6.0;
+  }
+}
+
 // Right-hand-side arguments of the step whose target is s.  BDF order by distance from
 // the chain end the sweep starts at (wormlike_MDE_2dcomp_mod.f90:325-343 BDF1, :484-502
 // BDF2, :584-608 BDF3; backward :1046-1064, :1138-1157, :1234-1259); the field is the
@@ -296,19 +313,7 @@ void rhs_args(wlc_handle* h, const SweepCtx& sw, int s, wlc::RhsArgs* ra, int* o
       ra->hM[j] = 0;
     }
   }
-  if (*order == 1) {
-    ra->a[0] = 1.0; ra->a[1] = 0.0; ra->a[2] = 0.0;
-    ra->b[0] = 1.0; ra->b[1] = 0.0; ra->b[2] = 0.0;
-    *cdelta = 1.0;
-  } else if (*order == 2) {
-    ra->a[0] = 2.0; ra->a[1] = -0.5; ra->a[2] = 0.0;
-    ra->b[0] = 2.0; ra->b[1] = -1.0; ra->b[2] = 0.0;
-    *cdelta = 1.5;
-  } else {
-    ra->a[0] = 3.0; ra->a[1] = -1.5; ra->a[2] = 1.0 / 3.0;
-    ra->b[0] = 3.0; ra->b[1] = -3.0; ra->b[2] = 1.0;
-    *cdelta = 11.0 / 6.0;
-  }
+  bdf_coeffs(*order, ra->a, ra->b, cdelta);
   ra->ds = h->ds;
   ra->w = h->w + (long long)blk * h->hs;
   ra->zero = h->zero;
@@ -342,7 +347,8 @@ int step_solve(wlc_handle* h, const SweepCtx& sw, int s) {
   sa.spec = sw.spec;
   sa.M = h->Mb[blk];
   sa.L = h->Lb[blk];
-  sa.TX = h->TX[blk];
+  const wlc::ZOpts zo = wlc::zsolve_opts(h->p.nz, h->Lb[blk], h->p.spectrum_fp32);
+  sa.TX = zo.tx;
   sa.cdelta = order == 1 ? 1.0 : (order == 2 ? 1.5 : 11.0 / 6.0);
   sa.sigma = dir > 0 ? 1.0 : -1.0;
   sa.ds = h->ds;
@@ -350,9 +356,9 @@ int step_solve(wlc_handle* h, const SweepCtx& sw, int s) {
   const double tp = 2.0 * M_PI;
   for (int a = 0; a < 3; ++a) sa.kfac[a] = tp / h->p.box[a];
   sa.solve = 1;
-  sa.generic = h->generic_zsolve;
-  sa.occ = h->zs_occ;
-  sa.split = h->zs_split;
+  sa.generic = zo.generic;
+  sa.occ = zo.occ;
+  sa.split = zo.split;
   TimedLaunch t(h, 1, sw.st);
   HIPCHK(wlc::launch_zsolve(h->geom, sa, sw.st));
   return WLC_OK;
@@ -676,7 +682,113 @@ int backward_concurrent(wlc_handle* h) {
   return WLC_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// One contour step on caller device buffers in the dense layout (z-plane stride nx ny, field
+// stride nz nx ny): kernels A, B and C of the sweep on the target s of direction dir, with
+// the chain's block rules (target block's field, Lmax and lambda; history zero-padded or
+// truncated to the target block's harmonics).  `spec` (half spectrum of the target's M
+// harmonics) and `zero` (one zero field) are the scratch.
+
+int check_step_args(const wlc_params* p, int dir, int s, const double* const hist[3],
+                    const double* w, const double* out, int* order) {
+  if (dir != 1 && dir != -1) return fail(WLC_EINVAL, "dir must be +1 or -1");
+  if (!hist || !w || !out) return fail(WLC_EINVAL, "NULL argument");
+  if ((dir > 0 && (s < 1 || s > p->Ns)) || (dir < 0 && (s < 0 || s > p->Ns - 1)))
+    return fail(WLC_EINVAL, "target s out of range for this direction");
+  *order = dir > 0 ? std::min(s, 3) : std::min(p->Ns - s, 3);
+  for (int j = 0; j < *order; ++j)
+    if (!hist[j]) return fail(WLC_EINVAL, "a history slice within the BDF order is NULL");
+  return WLC_OK;
+}
+
+int params_block_of(const wlc_params& p, int s) {
+  int st = 0;
+  for (int b = 0; b < p.nblk; ++b) {
+    st += p.Nblk[b];
+    if (s <= st) return b;
+  }
+  return p.nblk - 1;
+}
+
+int queue_dense_step(const wlc_params& p, const wlc::Geom& g, int dir, int s,
+                     int order, const double* const hist[3], const double* w, double* out,
+                     double2* spec, const double* zero, hipStream_t st) {
+  const double ds = 1.0 / p.Ns;
+  const int blk = params_block_of(p, s);
+  const int M = (p.Lmax[blk] + 1) * (p.Lmax[blk] + 1);
+  wlc::RhsArgs ra{};
+  double cdelta;
+  bdf_coeffs(order, ra.a, ra.b, &cdelta);
+  for (int j = 0; j < 3; ++j) {
+    const int sh = s - dir * (j + 1);
+    if (j < order) {
+      const int hb = params_block_of(p, sh);
+      ra.h[j] = hist[j];
+      ra.hM[j] = (p.Lmax[hb] + 1) * (p.Lmax[hb] + 1);
+    } else {
+      ra.h[j] = nullptr;
+      ra.hM[j] = 0;
+    }
+  }
+  ra.ds = ds;
+  ra.w = w + (long long)blk * g.hs;
+  ra.zero = zero;
+  ra.M = M;
+  ra.spec = spec;
+  HIPCHK(wlc::launch_rhs_fft(g, ra, order, st));
+  wlc::SolveArgs sa{};
+  sa.spec = spec;
+  sa.M = M;
+  sa.L = p.Lmax[blk];
+  const wlc::ZOpts o = wlc::zsolve_opts(p.nz, p.Lmax[blk], p.spectrum_fp32);
+  sa.TX = o.tx;
+  sa.cdelta = cdelta;
+  sa.sigma = dir > 0 ? 1.0 : -1.0;
+  sa.ds = ds;
+  sa.dslam = ds * p.lambda[blk];
+  for (int a = 0; a < 3; ++a) sa.kfac[a] = 2.0 * M_PI / p.box[a];
+  sa.solve = 1;
+  sa.generic = o.generic;
+  sa.occ = o.occ;
+  sa.split = o.split;
+  HIPCHK(wlc::launch_zsolve(g, sa, st));
+  wlc::InvArgs ia{};
+  ia.spec = spec;
+  ia.M = M;
+  ia.scale = 1.0 / ((double)p.nx * p.ny * p.nz);
+  ia.out = out;
+  HIPCHK(wlc::launch_ifft_store(g, ia, st));
+  return WLC_OK;
+}
+
+// orientation tables built on the host and checked against the compiled-in ones, once per
+// process and once per device
+int ensure_tables(int device) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  if (!g_tab_built) {
+    std::string err = wlc::build_orientation_tables(wlc::kMaxL, &g_tab);
+    if (!err.empty()) return fail(WLC_EINVAL, "orientation tables: " + err);
+    g_tab_built = true;
+  }
+  if ((int)g_tab_uploaded.size() <= device) g_tab_uploaded.resize(device + 1, 0);
+  if (!g_tab_uploaded[device]) {
+    if (wlc::upload_orientation_tables(g_tab) != hipSuccess)
+      return fail(WLC_EHIP, "compiled-in orientation tables differ from the host set-up (regenerate wlc_tables.inc)");
+    g_tab_uploaded[device] = 1;
+  }
+  return WLC_OK;
+}
+
 }  // namespace
+
+namespace wlc {
+int host_fail(int code, const std::string& msg) { return fail(code, msg); }
+int validate_params(const wlc_params* p) { return validate(p); }
+void simpson_weights(int n, double ds, std::vector<double>* w) { contour_weights(n, ds, w); }
+int zsolve_tx(int nz) { return pick_tx(nz); }
+int orientation_tables_ready(int device) { return ensure_tables(device); }
+void bdf(int order, double a[3], double b[3], double* cdelta) { bdf_coeffs(order, a, b, cdelta); }
+}  // namespace wlc
 
 extern "C" {
 
@@ -697,27 +809,14 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
   }
   wlc_handle* h = new wlc_handle();
   h->p = *p;
-  if (const char* ge = std::getenv("WLC_GENERIC_ZSOLVE")) h->generic_zsolve = std::atoi(ge) != 0;
-  if (const char* oc = std::getenv("WLC_ZSOLVE_OCC")) h->zs_occ = std::atoi(oc) == 2 ? 2 : 1;
-  if (const char* sp = std::getenv("WLC_ZSOLVE_SPLIT")) {
-    const int v = std::atoi(sp);
-    h->zs_split = (v == 1 || v == 2) ? v : 0;
-  }
   if (const char* fu = std::getenv("WLC_FUSE")) h->fuse = std::atoi(fu) != 0;
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
     return fail(WLC_EHIP, "no HIP device");
   }
-  {
-    std::lock_guard<std::mutex> lk(g_tab_mu);
-    if ((int)g_tab_uploaded.size() <= h->device) g_tab_uploaded.resize(h->device + 1, 0);
-    if (!g_tab_uploaded[h->device]) {
-      if (wlc::upload_orientation_tables(g_tab) != hipSuccess) {
-        delete h;
-        return fail(WLC_EHIP, "compiled-in orientation tables differ from the host set-up (regenerate wlc_tables.inc)");
-      }
-      g_tab_uploaded[h->device] = 1;
-    }
+  if ((rc = ensure_tables(h->device))) {
+    delete h;
+    return rc;
   }
   h->N = (long long)p->nx * p->ny * p->nz;
   // padded real-space strides (see layout notes in wlc_kernels.h)
@@ -734,11 +833,6 @@ int wlc_mde_create(const wlc_params* p, wlc_handle** out) {
     h->start[b] = st;
     st += p->Nblk[b];
     h->Mmax = std::max(h->Mmax, h->Mb[b]);
-    h->TX[b] = pick_tx(p->nz);
-    if (const char* tx = std::getenv("WLC_ZSOLVE_TX")) {  // measurement override
-      const int v = std::atoi(tx);
-      if (v >= 1 && v * p->nz <= 256) h->TX[b] = v;
-    }
     std::vector<double> cw;
     contour_weights(p->Nblk[b], h->ds, &cw);
     h->cw.push_back(cw);
@@ -1165,6 +1259,19 @@ int wlc_test_solve(int L, int nk, const double* kvec_host, double cdelta, double
   (void)hipFree(dk);
   (void)hipFree(dv);
   return rc;
+}
+
+int wlc_mde_step(wlc_handle* h, int dir, int s, const double* const hist_dev[3],
+                 const double* w_dev, double* out_dev) {
+  if (!h) return fail(WLC_EINVAL, "NULL handle");
+  int order = 0;
+  int rc = check_step_args(&h->p, dir, s, hist_dev, w_dev, out_dev, &order);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(h->device));
+  wlc::Geom g = h->geom;
+  g.zs = (long long)h->p.nx * h->p.ny;
+  g.hs = (long long)h->p.nz * g.zs;
+  return queue_dense_step(h->p, g, dir, s, order, hist_dev, w_dev, out_dev, h->spec, h->zero, h->st);
 }
 
 }  // extern "C"

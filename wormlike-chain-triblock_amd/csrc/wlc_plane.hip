@@ -260,8 +260,7 @@ __device__ __forceinline__ void rhs_to_lds(double* pk, const Geom& g, const RhsA
 // A: per (z-plane, harmonic): BDF right-hand side from the history slices, forward
 // x and y passes
 template <int NX, int NY, int MAXE, int ORDER, class S>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1)))
-k_rhs_fft_xy(Geom g, RhsArgs a) {
+__device__ __forceinline__ void k_rhs_fft_xy_body(const Geom& g, const RhsArgs& a) {
   extern __shared__ double2 lds[];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -271,9 +270,22 @@ k_rhs_fft_xy(Geom g, RhsArgs a) {
   forward_xy<NX, NY, MAXE>(lds, g, reinterpret_cast<S*>(a.spec), i, z);
 }
 
+template <int NX, int NY, int MAXE, int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1))) k_rhs_fft_xy(Geom g, RhsArgs a) {
+  k_rhs_fft_xy_body<NX, NY, MAXE, ORDER, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int NX, int NY, int MAXE, int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1))) k_rhs_fft_xy_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_rhs_fft_xy_body<NX, NY, MAXE, ORDER, S>(j.g, j.a);
+}
+
 // C: per (z-plane, harmonic): inverse y and x (C2R) passes, 1/N, store of q(s)
 template <int NX, int NY, int MAXE, class S>
-__global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
+__device__ __forceinline__ void k_ifft_xy_body(const Geom& g, const InvArgs& a) {
   extern __shared__ double2 lds[];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -288,14 +300,26 @@ __global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
   }
 }
 
+template <int NX, int NY, int MAXE, class S>
+__global__ void __launch_bounds__(kThreads) k_ifft_xy(Geom g, InvArgs a) {
+  k_ifft_xy_body<NX, NY, MAXE, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int NX, int NY, int MAXE, class S>
+__global__ void __launch_bounds__(kThreads) k_ifft_xy_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_ifft_xy_body<NX, NY, MAXE, S>(j.g, j.c);
+}
+
 // C of step s fused with A of the next step: per (z-plane, harmonic i <
 // max(M(s), M(next))).  For i < M(s) the solved plane is inverted and q_i(s) stored;
 // it stays in LDS as the newest history plane of the next right-hand side (zero for
 // i >= M(s): a junction that grows the basis).  For i < M(next) the right-hand side
 // is formed in place and transformed forward into the same spectrum plane.
 template <int NX, int NY, int MAXE, int ORDER, class S>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1)))
-k_inv_rhs(Geom g, InvArgs c, RhsArgs a) {
+__device__ __forceinline__ void k_inv_rhs_body(const Geom& g, const InvArgs& c, const RhsArgs& a) {
   extern __shared__ double2 lds[];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -306,6 +330,19 @@ k_inv_rhs(Geom g, InvArgs c, RhsArgs a) {
   rhs_to_lds<NX, NY, ORDER, true>(reinterpret_cast<double*>(lds), g, a, i, plane, inv, c.scale,
                               c.out + i * g.hs + plane, inv, rhs);
   if (rhs) forward_xy<NX, NY, MAXE>(lds, g, reinterpret_cast<S*>(a.spec), i, z);
+}
+
+template <int NX, int NY, int MAXE, int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1))) k_inv_rhs(Geom g, InvArgs c, RhsArgs a) {
+  k_inv_rhs_body<NX, NY, MAXE, ORDER, S>(g, c, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int NX, int NY, int MAXE, int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NX > 0 ? 2 : 1))) k_inv_rhs_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_inv_rhs_body<NX, NY, MAXE, ORDER, S>(j.g, j.c, j.a);
 }
 
 namespace {
@@ -319,22 +356,23 @@ int pick_maxe(long long elems, int nthr) {
 }
 }  // namespace
 
-template <int NX, int NY, int MAXE, class S>
+// B: the batched twin (jobs per blockIdx.z) instead of the single launch
+template <int NX, int NY, int MAXE, class S, bool B>
 static void* rhs_kernel(int order) {
   switch (order) {
-    case 1: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 1, S>;
-    case 2: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 2, S>;
-    case 3: return (void*)k_rhs_fft_xy<NX, NY, MAXE, 3, S>;
+    case 1: return B ? (void*)k_rhs_fft_xy_b<NX, NY, MAXE, 1, S> : (void*)k_rhs_fft_xy<NX, NY, MAXE, 1, S>;
+    case 2: return B ? (void*)k_rhs_fft_xy_b<NX, NY, MAXE, 2, S> : (void*)k_rhs_fft_xy<NX, NY, MAXE, 2, S>;
+    case 3: return B ? (void*)k_rhs_fft_xy_b<NX, NY, MAXE, 3, S> : (void*)k_rhs_fft_xy<NX, NY, MAXE, 3, S>;
   }
   return nullptr;
 }
 
-template <int NX, int NY, int MAXE, class S>
+template <int NX, int NY, int MAXE, class S, bool B>
 static void* inv_rhs_kernel(int order) {
   switch (order) {
-    case 1: return (void*)k_inv_rhs<NX, NY, MAXE, 1, S>;
-    case 2: return (void*)k_inv_rhs<NX, NY, MAXE, 2, S>;
-    case 3: return (void*)k_inv_rhs<NX, NY, MAXE, 3, S>;
+    case 1: return B ? (void*)k_inv_rhs_b<NX, NY, MAXE, 1, S> : (void*)k_inv_rhs<NX, NY, MAXE, 1, S>;
+    case 2: return B ? (void*)k_inv_rhs_b<NX, NY, MAXE, 2, S> : (void*)k_inv_rhs<NX, NY, MAXE, 2, S>;
+    case 3: return B ? (void*)k_inv_rhs_b<NX, NY, MAXE, 3, S> : (void*)k_inv_rhs<NX, NY, MAXE, 3, S>;
   }
   return nullptr;
 }
@@ -362,46 +400,49 @@ bool plane_supported(int nx, int ny) {
 #include "wlc_plane128.h"
 namespace wlc {
 
-template <int ORDER, class S>
+template <int ORDER, class S, bool B>
 static void* p64_kernel(int kind) {
+  if (B) return kind == 0 ? (void*)p64::k_rhs64_b<ORDER, S> : (void*)p64::k_inv_rhs64_b<ORDER, S>;
   return kind == 0 ? (void*)p64::k_rhs64<ORDER, S> : (void*)p64::k_inv_rhs64<ORDER, S>;
 }
 
-template <int ORDER, class S>
+template <int ORDER, class S, bool B>
 static void* p128_kernel(int kind) {
+  if (B) return kind == 0 ? (void*)p128::k_rhs128_b<ORDER, S> : (void*)p128::k_inv_rhs128_b<ORDER, S>;
   return kind == 0 ? (void*)p128::k_rhs128<ORDER, S> : (void*)p128::k_inv_rhs128<ORDER, S>;
 }
 
 // kind 0: A (order), 1: C, 2: fused C + A (order of the next step); *threads = workgroup
-// size; S = storage type of the half spectrum
-template <class S>
+// size; S = storage type of the half spectrum; B: the batched twin
+template <class S, bool B>
 static void* pick_plane_kernel_t(const Geom& g, int kind, int order, size_t* lds, int* threads) {
   *lds = plane_lds_bytes(g);
   *threads = kThreads;
   if (g.nx == 64 && g.ny == 64) {  // static 32 KB plane
     *lds = 0;
-    if (kind == 1) return (void*)p64::k_inv64<S>;
+    if (kind == 1) return B ? (void*)p64::k_inv64_b<S> : (void*)p64::k_inv64<S>;
     switch (order) {
-      case 1: return p64_kernel<1, S>(kind);
-      case 2: return p64_kernel<2, S>(kind);
-      case 3: return p64_kernel<3, S>(kind);
+      case 1: return p64_kernel<1, S, B>(kind);
+      case 2: return p64_kernel<2, S, B>(kind);
+      case 3: return p64_kernel<3, S, B>(kind);
     }
     return nullptr;
   }
   if (g.nx == 128 && g.ny == 128) {  // 128 KB plane in dynamic LDS, 512 threads
     *lds = sizeof(double2) * p128::kPlane;
     *threads = p128::kThreads;
-    if (kind == 1) return (void*)p128::k_inv128<S>;
+    if (kind == 1) return B ? (void*)p128::k_inv128_b<S> : (void*)p128::k_inv128<S>;
     switch (order) {
-      case 1: return p128_kernel<1, S>(kind);
-      case 2: return p128_kernel<2, S>(kind);
-      case 3: return p128_kernel<3, S>(kind);
+      case 1: return p128_kernel<1, S, B>(kind);
+      case 2: return p128_kernel<2, S, B>(kind);
+      case 3: return p128_kernel<3, S, B>(kind);
     }
     return nullptr;
   }
-#define WLC_PICK(NX, NY, MAXE)                                                       \
-  (kind == 0 ? rhs_kernel<NX, NY, MAXE, S>(order)                                    \
-             : kind == 1 ? (void*)k_ifft_xy<NX, NY, MAXE, S> : inv_rhs_kernel<NX, NY, MAXE, S>(order))
+#define WLC_PICK(NX, NY, MAXE)                                                            \
+  (kind == 0 ? rhs_kernel<NX, NY, MAXE, S, B>(order)                                      \
+   : kind == 1 ? (B ? (void*)k_ifft_xy_b<NX, NY, MAXE, S> : (void*)k_ifft_xy<NX, NY, MAXE, S>) \
+               : inv_rhs_kernel<NX, NY, MAXE, S, B>(order))
   if (specialised(g)) return WLC_PICK(32, 32, 0);
   switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
     case 8: return WLC_PICK(0, 0, 8);
@@ -413,9 +454,13 @@ static void* pick_plane_kernel_t(const Geom& g, int kind, int order, size_t* lds
   return nullptr;
 }
 
-static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds, int* threads) {
-  return g.spec32 ? pick_plane_kernel_t<float2>(g, kind, order, lds, threads)
-                  : pick_plane_kernel_t<double2>(g, kind, order, lds, threads);
+static void* pick_plane_kernel(const Geom& g, int kind, int order, size_t* lds, int* threads,
+                               bool batch = false) {
+  if (batch)
+    return g.spec32 ? pick_plane_kernel_t<float2, true>(g, kind, order, lds, threads)
+                    : pick_plane_kernel_t<double2, true>(g, kind, order, lds, threads);
+  return g.spec32 ? pick_plane_kernel_t<float2, false>(g, kind, order, lds, threads)
+                  : pick_plane_kernel_t<double2, false>(g, kind, order, lds, threads);
 }
 
 static hipError_t launch_plane(const Geom& g, int kind, int order, int planes, void** args,
@@ -452,6 +497,43 @@ hipError_t launch_inv_rhs(const Geom& g, const InvArgs& c, const RhsArgs& a, int
   RhsArgs aa = a;
   void* args[] = {&gg, &cc, &aa};
   return launch_plane(g, 2, order, c.M > a.M ? c.M : a.M, args, st);
+}
+
+LaunchKey plane_key(const Geom& g, int kind, int order, int planes) {
+  size_t lds = 0;
+  int threads = kThreads;
+  const void* k = pick_plane_kernel(g, kind, order, &lds, &threads);
+  return LaunchKey{k, (unsigned)(g.nz * planes), 1u, (unsigned)threads, lds};
+}
+
+static hipError_t launch_plane_batch(const Geom& g, int kind, int order, int planes,
+                                     const PlaneJob* jobs, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  size_t lds = 0;
+  int threads = kThreads;
+  void* k = pick_plane_kernel(g, kind, order, &lds, &threads, true);
+  if (!k) return hipErrorInvalidValue;
+  if (lds) {
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  const PlaneJob* jp = jobs;
+  void* args[] = {&jp};
+  return hipLaunchKernel(k, dim3(g.nz * planes, 1, n), dim3(threads), args, lds, st);
+}
+
+hipError_t launch_rhs_fft_batch(const PlaneJob& rep, int order, const PlaneJob* jobs, int n,
+                                hipStream_t st) {
+  return launch_plane_batch(rep.g, 0, order, rep.a.M, jobs, n, st);
+}
+
+hipError_t launch_ifft_store_batch(const PlaneJob& rep, const PlaneJob* jobs, int n, hipStream_t st) {
+  return launch_plane_batch(rep.g, 1, 0, rep.c.M, jobs, n, st);
+}
+
+hipError_t launch_inv_rhs_batch(const PlaneJob& rep, int order, const PlaneJob* jobs, int n,
+                                hipStream_t st) {
+  return launch_plane_batch(rep.g, 2, order, rep.c.M > rep.a.M ? rep.c.M : rep.a.M, jobs, n, st);
 }
 
 // ---------------------------------------------------------------------------

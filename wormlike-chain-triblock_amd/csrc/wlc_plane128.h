@@ -296,7 +296,7 @@ __device__ __forceinline__ void rhs_rows(double2* buf, const RhsArgs& a, const G
 
 // A: right-hand side from the history slices, forward x and y passes
 template <int ORDER, class S>
-__global__ void __launch_bounds__(kThreads) k_rhs128(Geom g, RhsArgs a) {
+__device__ __forceinline__ void k_rhs128_body(const Geom& g, const RhsArgs& a) {
   extern __shared__ double2 buf[];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -304,9 +304,22 @@ __global__ void __launch_bounds__(kThreads) k_rhs128(Geom g, RhsArgs a) {
   xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z);
 }
 
+template <int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) k_rhs128(Geom g, RhsArgs a) {
+  k_rhs128_body<ORDER, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) k_rhs128_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_rhs128_body<ORDER, S>(j.g, j.a);
+}
+
 // C: inverse y and x passes, 1/N, store of q(s)
 template <class S>
-__global__ void __launch_bounds__(kThreads) k_inv128(Geom g, InvArgs c) {
+__device__ __forceinline__ void k_inv128_body(const Geom& g, const InvArgs& c) {
   extern __shared__ double2 buf[];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -318,9 +331,22 @@ __global__ void __launch_bounds__(kThreads) k_inv128(Geom g, InvArgs c) {
   rhs_rows<1, true>(buf, none, g, i, plane, true, c.scale, c.out + i * g.hs + plane, true, false);
 }
 
+template <class S>
+__global__ void __launch_bounds__(kThreads) k_inv128(Geom g, InvArgs c) {
+  k_inv128_body<S>(g, c);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <class S>
+__global__ void __launch_bounds__(kThreads) k_inv128_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_inv128_body<S>(j.g, j.c);
+}
+
 // C of step s fused with A of the next step
 template <int ORDER, class S>
-__global__ void __launch_bounds__(kThreads) k_inv_rhs128(Geom g, InvArgs c, RhsArgs a) {
+__device__ __forceinline__ void k_inv_rhs128_body(const Geom& g, const InvArgs& c, const RhsArgs& a) {
   extern __shared__ double2 buf[];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -333,6 +359,19 @@ __global__ void __launch_bounds__(kThreads) k_inv_rhs128(Geom g, InvArgs c, RhsA
   }
   rhs_rows<ORDER, true>(buf, a, g, i, plane, inv, c.scale, c.out + i * g.hs + plane, inv, rhs);
   if (rhs) xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z);
+}
+
+template <int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) k_inv_rhs128(Geom g, InvArgs c, RhsArgs a) {
+  k_inv_rhs128_body<ORDER, S>(g, c, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int ORDER, class S>
+__global__ void __launch_bounds__(kThreads) k_inv_rhs128_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_inv_rhs128_body<ORDER, S>(j.g, j.c, j.a);
 }
 
 }  // namespace p128

@@ -288,7 +288,7 @@ __device__ __forceinline__ void rhs_rows(double2* buf, const RhsArgs& a, const G
 
 // A: right-hand side from the history slices, forward x and y passes
 template <int ORDER, class S>
-__global__ void __launch_bounds__(256) k_rhs64(Geom g, RhsArgs a) {
+__device__ __forceinline__ void k_rhs64_body(const Geom& g, const RhsArgs& a) {
   __shared__ double2 buf[kLds];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -298,9 +298,22 @@ __global__ void __launch_bounds__(256) k_rhs64(Geom g, RhsArgs a) {
   xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z, w);
 }
 
+template <int ORDER, class S>
+__global__ void __launch_bounds__(256) k_rhs64(Geom g, RhsArgs a) {
+  k_rhs64_body<ORDER, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int ORDER, class S>
+__global__ void __launch_bounds__(256) k_rhs64_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_rhs64_body<ORDER, S>(j.g, j.a);
+}
+
 // C: inverse y and x passes, 1/N, store of q(s)
 template <class S>
-__global__ void __launch_bounds__(256) k_inv64(Geom g, InvArgs c) {
+__device__ __forceinline__ void k_inv64_body(const Geom& g, const InvArgs& c) {
   __shared__ double2 buf[kLds];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -314,9 +327,22 @@ __global__ void __launch_bounds__(256) k_inv64(Geom g, InvArgs c) {
   rhs_rows<1, true>(buf, none, g, i, plane, true, c.scale, c.out + i * g.hs + plane, true, false);
 }
 
+template <class S>
+__global__ void __launch_bounds__(256) k_inv64(Geom g, InvArgs c) {
+  k_inv64_body<S>(g, c);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <class S>
+__global__ void __launch_bounds__(256) k_inv64_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_inv64_body<S>(j.g, j.c);
+}
+
 // C of step s fused with A of the next step (see k_inv_rhs)
 template <int ORDER, class S>
-__global__ void __launch_bounds__(256) k_inv_rhs64(Geom g, InvArgs c, RhsArgs a) {
+__device__ __forceinline__ void k_inv_rhs64_body(const Geom& g, const InvArgs& c, const RhsArgs& a) {
   __shared__ double2 buf[kLds];
   int z, i;
   plane_of_block(g.nz, z, i);
@@ -331,6 +357,19 @@ __global__ void __launch_bounds__(256) k_inv_rhs64(Geom g, InvArgs c, RhsArgs a)
   }
   rhs_rows<ORDER, true>(buf, a, g, i, plane, inv, c.scale, c.out + i * g.hs + plane, inv, rhs);
   if (rhs) xy_forward(buf, reinterpret_cast<S*>(a.spec), g, i, z, w);
+}
+
+template <int ORDER, class S>
+__global__ void __launch_bounds__(256) k_inv_rhs64(Geom g, InvArgs c, RhsArgs a) {
+  k_inv_rhs64_body<ORDER, S>(g, c, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int ORDER, class S>
+__global__ void __launch_bounds__(256) k_inv_rhs64_b(const PlaneJob* __restrict__ jobs) {
+  const PlaneJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_inv_rhs64_body<ORDER, S>(j.g, j.c, j.a);
 }
 
 }  // namespace p64

@@ -51,7 +51,7 @@ __device__ __forceinline__ void z_transform(double2 (&v)[kM], double2* lds, cons
 // generic kernel B: any nz <= 256; TX columns of one ky row per workgroup, z transforms
 // through LDS eight harmonics at a time (block-wide Stockham passes)
 template <class S>
-__global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
+__device__ __forceinline__ void k_zsolve_body(const Geom& g, const SolveArgs& a) {
   extern __shared__ double2 lds[];
   const int nz = g.nz, ny = g.ny, nxh = g.nxh;
   const int TX = a.TX;  // TX * nz == blockDim.x
@@ -77,6 +77,19 @@ __global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
 #pragma unroll
     for (int i = 0; i < kM; ++i) spec_st(p + i * istride, v[i]);
   }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_zsolve(Geom g, SolveArgs a) {
+  k_zsolve_body<S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <class S>
+__global__ void __launch_bounds__(256) k_zsolve_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve_body<S>(j.g, j.a);
 }
 
 // kernel B for nz = 64: one wave owns one (kx, ky) column, lane = z; the 64-point z
@@ -190,7 +203,7 @@ __device__ __forceinline__ void z64_inverse(const double2 (&v)[kM], double2* lds
 }
 
 template <int OCC, class S>
-__global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
+__device__ __forceinline__ void k_zsolve64_body(const Geom& g, const SolveArgs& a) {
   __shared__ double2 lds[4 * kRegion];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ny = g.ny, nxh = g.nxh;
@@ -215,6 +228,19 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
     Solver<kL>::solve(v, kxv, kyv, kzv, a.cdelta, a.sigma, a.ds, a.dslam);
   }
   z64_inverse(v, lds, wv, lane, tw, tile, istride);
+}
+
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve64(Geom g, SolveArgs a) {
+  k_zsolve64_body<OCC, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve64_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve64_body<OCC, S>(j.g, j.a);
 }
 
 // ---------------------------------------------------------------------------
@@ -303,7 +329,7 @@ __device__ __forceinline__ void z64_inv_regs(double2 (&v)[N], double2* wl, int l
 }
 
 template <int OCC, class S>
-__global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
+__device__ __forceinline__ void k_zsolve64s_body(const Geom& g, const SolveArgs& a) {
   __shared__ double2 lds[4 * kRegionS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c = wv >> 1, part = wv & 1;
@@ -376,6 +402,19 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
 #pragma unroll
     for (int h = 0; h < PartHi::NH; ++h) spec_st(col + (long long)(PartHi::B0 + h) * istride + 4 * lane, v[h]);
   }
+}
+
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve64s(Geom g, SolveArgs a) {
+  k_zsolve64s_body<OCC, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve64s_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve64s_body<OCC, S>(j.g, j.a);
 }
 
 // ---------------------------------------------------------------------------
@@ -484,7 +523,7 @@ __device__ __forceinline__ void zt_inverse(const double2 (&v)[kNV], double2* gld
 }
 
 template <class S>
-__global__ void __launch_bounds__(512) k_zsolve64t(Geom g, SolveArgs a) {
+__device__ __forceinline__ void k_zsolve64t_body(const Geom& g, const SolveArgs& a) {
   extern __shared__ double2 lds[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int part = wv >> 2, c = wv & 3, gi = threadIdx.x & 255;
@@ -547,6 +586,19 @@ __global__ void __launch_bounds__(512) k_zsolve64t(Geom g, SolveArgs a) {
     }
   }
   zt_inverse(v, glds, wl, lane, gi, nh, real_col, tw, hp, istride, io);
+}
+
+template <class S>
+__global__ void __launch_bounds__(512) k_zsolve64t(Geom g, SolveArgs a) {
+  k_zsolve64t_body<S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <class S>
+__global__ void __launch_bounds__(512) k_zsolve64t_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve64t_body<S>(j.g, j.a);
 }
 
 // ---------------------------------------------------------------------------
@@ -674,7 +726,7 @@ __device__ __forceinline__ void z128_dit_store(const double2 (&v)[kM], double2* 
 }
 
 template <int OCC, class S>
-__global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
+__device__ __forceinline__ void k_zsolve128_body(const Geom& g, const SolveArgs& a) {
   __shared__ double2 lds[4 * kRegion128];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c = wv >> 1, p = wv & 1;
@@ -749,13 +801,26 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
   }
 }
 
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve128(Geom g, SolveArgs a) {
+  k_zsolve128_body<OCC, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve128_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve128_body<OCC, S>(j.g, j.a);
+}
+
 // nz = 128 with the degree split of k_zsolve64s: workgroup = one column, four waves
 // wv = 2 part + p (part 0: l <= kLS, part 1: l > kLS; p: z half).  The radix-2 step pairs
 // wv with wv ^ 1, the crossing tridiagonal chains pair wv with wv ^ 2 through the region of
 // the part-0 wave of the same z half.  Each wave holds about half the harmonics, so two
 // waves fit per SIMD at Lmax 7 and 8.
 template <int OCC, class S>
-__global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
+__device__ __forceinline__ void k_zsolve128s_body(const Geom& g, const SolveArgs& a) {
   __shared__ double2 lds[4 * kRegion128];
   static_assert(kXch + 2 <= kRegion128, "crossing chains fit a wave region");
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -859,6 +924,19 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
   }
 }
 
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve128s(Geom g, SolveArgs a) {
+  k_zsolve128s_body<OCC, S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <int OCC, class S>
+__global__ void __launch_bounds__(256, OCC) k_zsolve128s_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve128s_body<OCC, S>(j.g, j.a);
+}
+
 // generic kernel B (nz other than 64 and 128, or any nz with the generic kernel forced),
 // 4 <= Lmax <= 7, with the harmonics of each
 // k-point split by degree over two threads: thread t < TX nz holds l <= kLS of k-point
@@ -888,7 +966,7 @@ __device__ __forceinline__ void z_transform_n(double2 (&v)[N], double2* lds, con
 }
 
 template <class S>
-__global__ void __launch_bounds__(512) k_zsolve_split(Geom g, SolveArgs a) {
+__device__ __forceinline__ void k_zsolve_split_body(const Geom& g, const SolveArgs& a) {
   extern __shared__ double2 lds[];
   const int nz = g.nz, ny = g.ny, nxh = g.nxh;
   const int TX = a.TX;
@@ -943,49 +1021,78 @@ __global__ void __launch_bounds__(512) k_zsolve_split(Geom g, SolveArgs a) {
   }
 }
 
+template <class S>
+__global__ void __launch_bounds__(512) k_zsolve_split(Geom g, SolveArgs a) {
+  k_zsolve_split_body<S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <class S>
+__global__ void __launch_bounds__(512) k_zsolve_split_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve_split_body<S>(j.g, j.a);
+}
+
 }  // namespace per_l_<L>
 
 using namespace WLC_CAT(per_l_, WLC_L);
 
 namespace {
+// the kernel-B variant for (g, a): single-launch kernel, its batched twin, grid, block, LDS
+struct ZPlan {
+  const void* k1 = nullptr;
+  const void* kb = nullptr;
+  dim3 grid, block;
+  size_t lds = 0;
+};
+
 template <class S>
-hipError_t zsolve_launch(const Geom& g, const SolveArgs& a, hipStream_t st) {
+hipError_t zsolve_plan(const Geom& g, const SolveArgs& a, ZPlan* z) {
   if (a.M != kM) return hipErrorInvalidValue;
   if constexpr (std::is_same<S, double2>::value) {
     // one workgroup of eight waves per 4-kx tile; FP64 spectrum only: with the FP32 half
     // spectrum (2 KB tiles) the two-column kernel below is faster (64^3/L6: 69 vs 93 us)
     if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split == 2) {
-      const size_t lds = sizeof(double2) * (size_t)kTileLds;
-      hipError_t e = hipFuncSetAttribute((const void*)k_zsolve64t<S>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((k_zsolve64t<S>), dim3(g.nt * g.ny), dim3(512), lds, st, g, a);
-      return hipGetLastError();
+      z->k1 = (const void*)k_zsolve64t<S>;
+      z->kb = (const void*)k_zsolve64t_b<S>;
+      z->grid = dim3(g.nt * g.ny);
+      z->block = dim3(512);
+      z->lds = sizeof(double2) * (size_t)kTileLds;
+      return hipSuccess;
     }
   }
   if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
     // two workgroups per 4-kx tile
-    const int blocks = 2 * g.nt * g.ny;
-    hipLaunchKernelGGL((k_zsolve64s<2, S>), dim3(blocks), dim3(256), 0, st, g, a);
-    return hipGetLastError();
+    z->k1 = (const void*)k_zsolve64s<2, S>;
+    z->kb = (const void*)k_zsolve64s_b<2, S>;
+    z->grid = dim3(2 * g.nt * g.ny);
+    z->block = dim3(256);
+    return hipSuccess;
   }
   if (g.nz == 128 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
     // one workgroup (one column, harmonics split over two waves per z half) per column
-    const int blocks = 4 * g.nt * g.ny;
-    hipLaunchKernelGGL((k_zsolve128s<2, S>), dim3(blocks), dim3(256), 0, st, g, a);
-    return hipGetLastError();
+    z->k1 = (const void*)k_zsolve128s<2, S>;
+    z->kb = (const void*)k_zsolve128s_b<2, S>;
+    z->grid = dim3(4 * g.nt * g.ny);
+    z->block = dim3(256);
+    return hipSuccess;
   }
   if (g.nz == 128 && !a.generic) {
     // two workgroups (two columns each) per 4-kx tile
-    const int blocks = 2 * g.nt * g.ny;
-    hipLaunchKernelGGL((k_zsolve128<(kL >= 7 ? 1 : 2), S>), dim3(blocks), dim3(256), 0, st, g, a);
-    return hipGetLastError();
+    z->k1 = (const void*)k_zsolve128<(kL >= 7 ? 1 : 2), S>;
+    z->kb = (const void*)k_zsolve128_b<(kL >= 7 ? 1 : 2), S>;
+    z->grid = dim3(2 * g.nt * g.ny);
+    z->block = dim3(256);
+    return hipSuccess;
   }
   if (g.nz == 64 && !a.generic) {
     static_assert(kKxTile == 4, "one wave per kx of a tile");
-    if (a.occ == 2) hipLaunchKernelGGL((k_zsolve64<2, S>), dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
-    else hipLaunchKernelGGL((k_zsolve64<1, S>), dim3(g.nt, g.ny), dim3(256), 0, st, g, a);
-    return hipGetLastError();
+    z->k1 = a.occ == 2 ? (const void*)k_zsolve64<2, S> : (const void*)k_zsolve64<1, S>;
+    z->kb = a.occ == 2 ? (const void*)k_zsolve64_b<2, S> : (const void*)k_zsolve64_b<1, S>;
+    z->grid = dim3(g.nt, g.ny);
+    z->block = dim3(256);
+    return hipSuccess;
   }
   if (a.TX * g.nz > 256 || a.TX < 1) return hipErrorInvalidValue;
   if constexpr (kL >= 4 && kL <= 7) {
@@ -993,26 +1100,59 @@ hipError_t zsolve_launch(const Geom& g, const SolveArgs& a, hipStream_t st) {
       // two threads per k-point (degree split): 2 TX nz threads
       const int np = a.TX * g.nz;
       const size_t rows = (size_t)kHB * 2 * np, chains = (size_t)((np + 63) / 64) * kXch;
-      const size_t lds = sizeof(double2) * (rows > chains ? rows : chains);
-      hipError_t e = hipFuncSetAttribute((const void*)k_zsolve_split<S>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      dim3 grid((g.nxh + a.TX - 1) / a.TX, g.ny), block(2 * np);
-      hipLaunchKernelGGL(k_zsolve_split<S>, grid, block, lds, st, g, a);
-      return hipGetLastError();
+      z->k1 = (const void*)k_zsolve_split<S>;
+      z->kb = (const void*)k_zsolve_split_b<S>;
+      z->grid = dim3((g.nxh + a.TX - 1) / a.TX, g.ny);
+      z->block = dim3(2 * np);
+      z->lds = sizeof(double2) * (rows > chains ? rows : chains);
+      return hipSuccess;
     }
   }
-  const size_t lds = sizeof(double2) * (size_t)kHB * a.TX * g.nz;
-  hipError_t e = hipFuncSetAttribute((const void*)k_zsolve<S>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  dim3 grid((g.nxh + a.TX - 1) / a.TX, g.ny), block(a.TX * g.nz);
-  hipLaunchKernelGGL(k_zsolve<S>, grid, block, lds, st, g, a);
-  return hipGetLastError();
+  z->k1 = (const void*)k_zsolve<S>;
+  z->kb = (const void*)k_zsolve_b<S>;
+  z->grid = dim3((g.nxh + a.TX - 1) / a.TX, g.ny);
+  z->block = dim3(a.TX * g.nz);
+  z->lds = sizeof(double2) * (size_t)kHB * a.TX * g.nz;
+  return hipSuccess;
+}
+
+hipError_t plan_of(const Geom& g, const SolveArgs& a, ZPlan* z) {
+  return g.spec32 ? zsolve_plan<float2>(g, a, z) : zsolve_plan<double2>(g, a, z);
 }
 }  // namespace
 
 hipError_t WLC_CAT(zsolve_launch_L, WLC_L)(const Geom& g, const SolveArgs& a, hipStream_t st) {
-  return g.spec32 ? zsolve_launch<float2>(g, a, st) : zsolve_launch<double2>(g, a, st);
+  ZPlan z;
+  hipError_t e = plan_of(g, a, &z);
+  if (e != hipSuccess) return e;
+  if (z.lds) {
+    e = hipFuncSetAttribute(z.k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)z.lds);
+    if (e != hipSuccess) return e;
+  }
+  Geom gg = g;
+  SolveArgs aa = a;
+  void* args[] = {&gg, &aa};
+  return hipLaunchKernel(z.k1, z.grid, z.block, args, z.lds, st);
+}
+
+hipError_t WLC_CAT(zsolve_batch_L, WLC_L)(const ZJob& rep, const ZJob* jobs, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  ZPlan z;
+  hipError_t e = plan_of(rep.g, rep.a, &z);
+  if (e != hipSuccess) return e;
+  if (z.lds) {
+    e = hipFuncSetAttribute(z.kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)z.lds);
+    if (e != hipSuccess) return e;
+  }
+  const ZJob* jp = jobs;
+  void* args[] = {&jp};
+  return hipLaunchKernel(z.kb, dim3(z.grid.x, z.grid.y, n), z.block, args, z.lds, st);
+}
+
+LaunchKey WLC_CAT(zsolve_key_L, WLC_L)(const Geom& g, const SolveArgs& a) {
+  ZPlan z;
+  if (plan_of(g, a, &z) != hipSuccess) return LaunchKey{nullptr, 0, 0, 0, 0};
+  return LaunchKey{z.k1, z.grid.x, z.grid.y, z.block.x, z.lds};
 }
 
 namespace WLC_CAT(per_l_, WLC_L) {
