@@ -233,6 +233,9 @@ void wlc_batch_destroy(wlc_batch* b);
  * accumulation, FP32 master weights and Adam. */
 typedef struct wlc_unet wlc_unet;
 int wlc_unet_create(int n, int batch, unsigned long long seed, wlc_unet** out);
+/* new grid edge (multiple of 8) and batch for the next calls; parameters and Adam state kept
+ * (the network is fully convolutional) */
+int wlc_unet_resize(wlc_unet* u, int n, int batch);
 /* trainable parameters (padded = 1: the flat layout's length, including zero padding) */
 long long wlc_unet_num_params(const wlc_unet* u, int padded);
 /* layer i: info[8] = cin, cout (padded), cin, cout (real), level, relu, weight offset, bias

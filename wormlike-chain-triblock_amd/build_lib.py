@@ -40,6 +40,21 @@ def _units():
     return units
 
 
+def _stale(unit):
+    """True when the object is missing or older than its source or any header of csrc/ and
+    include/ (incremental development builds; build() without incremental recompiles all)."""
+    src, obj, _ = unit
+    o = os.path.join(BUILD, obj)
+    if not os.path.exists(o):
+        return True
+    t = os.path.getmtime(o)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                                        if f.endswith((".h", ".inc"))]
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    deps += [os.path.join(inc, f) for f in os.listdir(inc)]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
 def _compile(unit):
     src, obj, extra = unit
     cmd = [HIPCC] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD, obj)]
@@ -69,13 +84,15 @@ def gen_tables():
             f.write(new)
 
 
-def build(jobs=None, verbose=True):
+def build(jobs=None, verbose=True, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
     gen_tables()
     units = _units()
-    jobs = jobs or min(len(units), max(1, min(os.cpu_count() or 1, 16)))
+    todo = [u for u in units if _stale(u)] if incremental else units
+    jobs = jobs or min(max(1, len(todo)), max(1, min(os.cpu_count() or 1, 16)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, units))
+        list(ex.map(_compile, todo))
+    objs = [u[1] for u in units]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + \
         [os.path.join(BUILD, o) for o in objs]
@@ -89,5 +106,5 @@ def build(jobs=None, verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

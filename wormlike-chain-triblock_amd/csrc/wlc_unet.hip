@@ -409,7 +409,8 @@ struct wlc_unet {
   float* dy[11] = {};     // their gradients
   bf16* dyb[11] = {};     // bf16 copies (matrix-core operand)
   float* target = nullptr;
-  std::vector<void*> allocs;
+  std::vector<void*> allocs;  // parameters and optimiser state
+  std::vector<void*> acts;    // activations, gradients and scratch of the current grid and batch
   Vol vol(int level) const { return Vol{B, n >> level, n >> level, n >> level}; }
 };
 
@@ -424,24 +425,64 @@ int ufail(int code, const std::string& m) { return wlc::host_fail(code, m); }
   } while (0)
 
 template <class T>
-int ualloc(wlc_unet* u, T** ptr, long long count) {
+int ualloc(std::vector<void*>& list, T** ptr, long long count) {
   void* q = nullptr;
   if (hipMalloc(&q, sizeof(T) * (size_t)std::max<long long>(count, 1)) != hipSuccess)
     return ufail(WLC_ENOMEM, "device allocation failed (warm-start network)");
   if (hipMemset(q, 0, sizeof(T) * (size_t)std::max<long long>(count, 1)) != hipSuccess)
     return ufail(WLC_EHIP, "device initialisation failed (warm-start network)");
-  u->allocs.push_back(q);
+  list.push_back(q);
   *ptr = static_cast<T*>(q);
   return WLC_OK;
+}
+
+void free_acts(wlc_unet* u) {
+  for (void* q : u->acts) (void)hipFree(q);
+  u->acts.clear();
 }
 
 void unet_free(wlc_unet* u) {
   if (!u) return;
   (void)hipSetDevice(u->device);
   if (u->st) (void)hipStreamSynchronize(u->st);
+  free_acts(u);
   for (void* q : u->allocs) (void)hipFree(q);
   if (u->st) (void)hipStreamDestroy(u->st);
   delete u;
+}
+
+// activation, gradient and scratch buffers for grid edge n and batch B
+int alloc_acts(wlc_unet* u, int n, int B) {
+  free_acts(u);
+  u->n = n;
+  u->B = B;
+  int rc;
+#define UA(ptr, cnt) if ((rc = ualloc(u->acts, &(ptr), (cnt)))) { free_acts(u); return rc; }
+  long long maxpart = 0;
+  for (const Layer& l : u->layers) {
+    const long long nw = (long long)l.cout * 27 * l.cin;
+    const long long V = u->vol(l.level).voxels();
+    const long long nch = std::min<long long>(64, std::max<long long>(1, V / 4096));
+    maxpart = std::max(maxpart, nch * nw);
+  }
+  UA(u->part, maxpart); UA(u->lpart, 256);
+  const long long V0 = u->vol(0).voxels(), V1 = u->vol(1).voxels(), V2 = u->vol(2).voxels(), V3 = u->vol(3).voxels();
+  UA(u->x0, V0 * 16); UA(u->a0a, V0 * 32); UA(u->s0, V0 * 32); UA(u->p1, V1 * 32); UA(u->a1a, V1 * 64);
+  UA(u->s1, V1 * 64); UA(u->p2, V2 * 64); UA(u->a2a, V2 * 128); UA(u->s2, V2 * 128); UA(u->p3, V3 * 128);
+  UA(u->bt, V3 * 128); UA(u->c2, V2 * 256); UA(u->d2, V2 * 64); UA(u->c1, V1 * 128); UA(u->d1, V1 * 32);
+  UA(u->c0, V0 * 64); UA(u->d0, V0 * 32);
+  UA(u->ga0a, V0 * 32); UA(u->gs0, V0 * 32); UA(u->gp1, V1 * 32); UA(u->ga1a, V1 * 64); UA(u->gs1, V1 * 64);
+  UA(u->gp2, V2 * 64); UA(u->ga2a, V2 * 128); UA(u->gs2, V2 * 128); UA(u->gp3, V3 * 128); UA(u->gbt, V3 * 128);
+  UA(u->gc2, V2 * 256); UA(u->gd2, V2 * 64); UA(u->gc1, V1 * 128); UA(u->gd1, V1 * 32); UA(u->gc0, V0 * 64);
+  UA(u->gd0, V0 * 32);
+  for (int li = 0; li < 11; ++li) {
+    const Layer& l = u->layers[li];
+    const long long V = u->vol(l.level).voxels();
+    UA(u->y[li], V * l.cout); UA(u->dy[li], V * l.cout); UA(u->dyb[li], V * l.cout);
+  }
+  UA(u->target, V0 * 3);
+#undef UA
+  return WLC_OK;
 }
 
 // y = conv(X) (+ bias) of layer li on level lv; X has l.cin channels
@@ -605,33 +646,16 @@ int wlc_unet_create(int n, int batch, unsigned long long seed, wlc_unet** out) {
   const Layer& last = u->layers.back();
   u->np = last.boff + last.cout;
   int rc;
-#define UA(ptr, cnt) if ((rc = ualloc(u, &(ptr), (cnt)))) { unet_free(u); return rc; }
+#define UA(ptr, cnt) if ((rc = ualloc(u->allocs, &(ptr), (cnt)))) { unet_free(u); return rc; }
   UA(u->p, u->np); UA(u->m, u->np); UA(u->v, u->np); UA(u->g, u->np); UA(u->pb, u->np); UA(u->mask, u->np);
-  long long maxw = 0, maxpart = 0;
-  for (const Layer& l : u->layers) {
-    const long long nw = (long long)l.cout * 27 * l.cin;
-    maxw = std::max(maxw, nw);
-    const long long V = u->vol(l.level).voxels();
-    const long long nch = std::min<long long>(64, std::max<long long>(1, V / 4096));
-    maxpart = std::max(maxpart, nch * nw);
-  }
-  UA(u->wflip, maxw); UA(u->part, maxpart); UA(u->lpart, 256);
-  const long long V0 = u->vol(0).voxels(), V1 = u->vol(1).voxels(), V2 = u->vol(2).voxels(), V3 = u->vol(3).voxels();
-  UA(u->x0, V0 * 16); UA(u->a0a, V0 * 32); UA(u->s0, V0 * 32); UA(u->p1, V1 * 32); UA(u->a1a, V1 * 64);
-  UA(u->s1, V1 * 64); UA(u->p2, V2 * 64); UA(u->a2a, V2 * 128); UA(u->s2, V2 * 128); UA(u->p3, V3 * 128);
-  UA(u->bt, V3 * 128); UA(u->c2, V2 * 256); UA(u->d2, V2 * 64); UA(u->c1, V1 * 128); UA(u->d1, V1 * 32);
-  UA(u->c0, V0 * 64); UA(u->d0, V0 * 32);
-  UA(u->ga0a, V0 * 32); UA(u->gs0, V0 * 32); UA(u->gp1, V1 * 32); UA(u->ga1a, V1 * 64); UA(u->gs1, V1 * 64);
-  UA(u->gp2, V2 * 64); UA(u->ga2a, V2 * 128); UA(u->gs2, V2 * 128); UA(u->gp3, V3 * 128); UA(u->gbt, V3 * 128);
-  UA(u->gc2, V2 * 256); UA(u->gd2, V2 * 64); UA(u->gc1, V1 * 128); UA(u->gd1, V1 * 32); UA(u->gc0, V0 * 64);
-  UA(u->gd0, V0 * 32);
-  for (int li = 0; li < 11; ++li) {
-    const Layer& l = u->layers[li];
-    const long long V = u->vol(l.level).voxels();
-    UA(u->y[li], V * l.cout); UA(u->dy[li], V * l.cout); UA(u->dyb[li], V * l.cout);
-  }
-  UA(u->target, V0 * 3);
+  long long maxw = 0;
+  for (const Layer& l : u->layers) maxw = std::max(maxw, (long long)l.cout * 27 * l.cin);
+  UA(u->wflip, maxw);
 #undef UA
+  if ((rc = alloc_acts(u, n, batch))) {
+    unet_free(u);
+    return rc;
+  }
   // He-normal weights over the real fan-in, zero biases, zero padding entries; the output
   // layer starts small
   std::vector<float> hp((size_t)u->np, 0.f);
@@ -665,6 +689,14 @@ int wlc_unet_create(int n, int batch, unsigned long long seed, wlc_unet** out) {
 }
 
 void wlc_unet_destroy(wlc_unet* u) { unet_free(u); }
+
+int wlc_unet_resize(wlc_unet* u, int n, int batch) {
+  if (!u) return ufail(WLC_EINVAL, "NULL network");
+  if (n < 8 || n % 8 || batch < 1) return ufail(WLC_EINVAL, "grid edge must be a multiple of 8 and batch >= 1");
+  UCHK(hipSetDevice(u->device));
+  UCHK(hipStreamSynchronize(u->st));
+  return alloc_acts(u, n, batch);
+}
 
 long long wlc_unet_num_params(const wlc_unet* u, int padded) {
   if (!u) return WLC_EINVAL;

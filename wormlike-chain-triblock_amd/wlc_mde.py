@@ -72,7 +72,7 @@ EXPORTS = [
     "wlc_batch_create", "wlc_batch_size", "wlc_batch_launches",
     "wlc_batch_set_w", "wlc_batch_get_w", "wlc_batch_sweep", "wlc_batch_density",
     "wlc_batch_get_q", "wlc_batch_iterate", "wlc_batch_status", "wlc_batch_destroy",
-    "wlc_unet_create", "wlc_unet_num_params", "wlc_unet_layer", "wlc_unet_forward",
+    "wlc_unet_create", "wlc_unet_resize", "wlc_unet_num_params", "wlc_unet_layer", "wlc_unet_forward",
     "wlc_unet_gradients", "wlc_unet_adam", "wlc_unet_train_step", "wlc_unet_get",
     "wlc_unet_set_params", "wlc_unet_destroy", "wlc_test_conv3d",
 ]
@@ -130,6 +130,7 @@ def load(path=LIB_PATH):
     L.wlc_batch_destroy.restype = None
     fp = ctypes.POINTER(ctypes.c_float)
     L.wlc_unet_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    L.wlc_unet_resize.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.wlc_unet_num_params.argtypes = [vp, ctypes.c_int]
     L.wlc_unet_num_params.restype = ctypes.c_longlong
     L.wlc_unet_layer.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
@@ -428,6 +429,10 @@ class UNet:
             self.close()
         except Exception:
             pass
+
+    def resize(self, n, batch):
+        _check(_lib.wlc_unet_resize(self.h, int(n), int(batch)))
+        self.n, self.B = n, batch
 
     def num_params(self, padded=False):
         return _lib.wlc_unet_num_params(self.h, 1 if padded else 0)
