@@ -157,6 +157,18 @@ int wlc_mde_event_elapsed(wlc_handle* h, int slot_a, int slot_b, double* ms);
 int wlc_mde_step(wlc_handle* h, int dir, int s, const double* const hist_dev[3],
                  const double* w_dev, double* out_dev);
 
+/* wlc_mde_step without a handle and without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The chain is described by p
+ * (validated as in wlc_mde_create), the work is queued on `stream` (a hipStream_t, NULL =
+ * the null stream).  The scratch (FFT twiddles, one zero field, the half spectrum of the
+ * target's harmonics) lives in a fixed 128 MiB device array that each GPU the library runs
+ * on holds until the process ends; a step whose zero field plus half spectrum exceeds it
+ * (e.g. 64^3 with Lmax above 6) is rejected with WLC_EINVAL.  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (different streams or threads) race. */
+int wlc_mde_step_noalloc(const wlc_params* p, int dir, int s, const double* const hist_dev[3],
+                         const double* w_dev, double* out_dev, void* stream);
+
 /* ---- Batched SCFT: many independent instances on one GPU -------------------------------
  * The reference runs one SCFT per process (cal_scft, SCFT_control.f90:235-262: MDE_q,
  * MDE_qstar, density, free_energy, iterate; field update AB_iterate, SCFT_iterate.f90:27-333).

@@ -779,6 +779,48 @@ int ensure_tables(int device) {
   return WLC_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Scratch of wlc_mde_step_noalloc: a fixed device array, one copy per GPU, held until the
+// process ends.  Layout: FFT twiddles of the three axes at offset 0 (3 x 256 complex), one
+// zero field at kScrZero, the half spectrum at kScrZero + the zero field rounded up to
+// 256 bytes.  The host reaches each device's copy only through the address
+// hipGetSymbolAddress returns on that device.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+constexpr size_t kScrZero = 64 << 10;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scr_mu;
+std::vector<unsigned char*> g_scr_ptr;  // per device, nullptr until resolved
+
+int scratch_on(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if ((int)g_scr_ptr.size() <= device) g_scr_ptr.resize(device + 1, nullptr);
+  if (!g_scr_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(WLC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scr_ptr[device] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scr_ptr[device];
+  return WLC_OK;
+}
+
+// host twiddle tables per edge, kept for the life of the process (the source of the
+// per-call upload stays valid however the runtime stages the copy)
+std::mutex g_tw_mu;
+std::vector<std::vector<double2>> g_tw_host(257);
+
+const double2* host_twiddles(const wlc::AxisPlan& pl) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto& v = g_tw_host[pl.n];
+  if (v.empty()) {
+    v.resize(pl.n);
+    for (int t = 0; t < pl.n; ++t) v[t] = make_double2(pl.tw[2 * t], pl.tw[2 * t + 1]);
+  }
+  return v.data();
+}
+
 }  // namespace
 
 namespace wlc {
@@ -1272,6 +1314,57 @@ int wlc_mde_step(wlc_handle* h, int dir, int s, const double* const hist_dev[3],
   g.zs = (long long)h->p.nx * h->p.ny;
   g.hs = (long long)h->p.nz * g.zs;
   return queue_dense_step(h->p, g, dir, s, order, hist_dev, w_dev, out_dev, h->spec, h->zero, h->st);
+}
+
+int wlc_mde_step_noalloc(const wlc_params* p, int dir, int s, const double* const hist_dev[3],
+                         const double* w_dev, double* out_dev, void* stream) {
+  int rc = validate(p);
+  if (rc) return rc;
+  int order = 0;
+  if ((rc = check_step_args(p, dir, s, hist_dev, w_dev, out_dev, &order))) return rc;
+  const int nxh = p->nx / 2 + 1;
+  const int blk = params_block_of(*p, s);
+  const size_t M = (size_t)(p->Lmax[blk] + 1) * (p->Lmax[blk] + 1);
+  const size_t field = (size_t)p->nx * p->ny * p->nz;
+  const size_t zero_bytes = (sizeof(double) * field + 255) / 256 * 256;
+  const size_t spec_bytes = sizeof(double2) * M * p->ny * wlc::kx_tiles(nxh) * wlc::kKxTile * p->nz;
+  if (kScrZero + zero_bytes + spec_bytes > kScratchBytes)
+    return fail(WLC_EINVAL, "step too large for the fixed scratch (" + std::to_string(kScratchBytes >> 20) +
+                                " MiB: one zero field plus the half spectrum of the target's harmonics)");
+  wlc::AxisPlan plan[3];
+  const int edges[3] = {p->nx, p->ny, p->nz};
+  for (int a = 0; a < 3; ++a) {
+    std::string err = wlc::make_axis_plan(edges[a], &plan[a]);
+    if (!err.empty()) return fail(WLC_EINVAL, err);
+    if (plan[a].npass > 8) return fail(WLC_EINVAL, "too many FFT passes");
+  }
+  int device = 0;
+  HIPCHK(hipGetDevice(&device));
+  if ((rc = ensure_tables(device))) return rc;
+  unsigned char* scr = nullptr;
+  if ((rc = scratch_on(device, &scr))) return rc;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  double2* tw = reinterpret_cast<double2*>(scr);
+  double* zero = reinterpret_cast<double*>(scr + kScrZero);
+  double2* spec = reinterpret_cast<double2*>(scr + kScrZero + zero_bytes);
+  wlc::Geom g{};
+  g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nxh = nxh;
+  g.nt = wlc::kx_tiles(nxh);
+  g.spec32 = p->spectrum_fp32;
+  g.zs = (long long)p->nx * p->ny;
+  g.hs = (long long)p->nz * g.zs;
+  wlc::AxisDev* ax[3] = {&g.ax, &g.ay, &g.az};
+  for (int a = 0; a < 3; ++a) {
+    ax[a]->n = plan[a].n;
+    ax[a]->npass = plan[a].npass;
+    for (int i = 0; i < 8; ++i) ax[a]->radix[i] = i < plan[a].npass ? plan[a].radix[i] : 0;
+    ax[a]->tw = tw + 256 * a;
+    HIPCHK(hipMemcpyAsync(tw + 256 * a, host_twiddles(plan[a]), sizeof(double2) * plan[a].n,
+                          hipMemcpyHostToDevice, st));
+  }
+  // the array keeps the previous call's contents: the zero field is cleared every call
+  HIPCHK(wlc::launch_fill(zero, 0.0, (long long)field, st));
+  return queue_dense_step(*p, g, dir, s, order, hist_dev, w_dev, out_dev, spec, zero, st);
 }
 
 }  // extern "C"

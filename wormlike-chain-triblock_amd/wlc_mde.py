@@ -69,7 +69,7 @@ EXPORTS = [
     "wlc_mde_stream", "wlc_mde_set_timing", "wlc_mde_get_timing", "wlc_mde_event_record",
     "wlc_mde_event_elapsed", "wlc_mde_destroy", "wlc_mde_last_error", "wlc_test_solve",
     "wlc_mde_set_spectrum_fp32", "wlc_mde_forward_stored", "wlc_mde_step",
-    "wlc_batch_create", "wlc_batch_size", "wlc_batch_launches",
+    "wlc_mde_step_noalloc", "wlc_batch_create", "wlc_batch_size", "wlc_batch_launches",
     "wlc_batch_set_w", "wlc_batch_get_w", "wlc_batch_sweep", "wlc_batch_density",
     "wlc_batch_get_q", "wlc_batch_iterate", "wlc_batch_status", "wlc_batch_destroy",
     "wlc_unet_create", "wlc_unet_resize", "wlc_unet_num_params", "wlc_unet_layer", "wlc_unet_forward",
@@ -115,6 +115,8 @@ def load(path=LIB_PATH):
     L.wlc_mde_destroy.restype = None
     L.wlc_mde_last_error.restype = ctypes.c_char_p
     L.wlc_mde_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp), vp, vp]
+    L.wlc_mde_step_noalloc.argtypes = [ctypes.POINTER(WlcParams), ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(vp), vp, vp, vp]
     L.wlc_batch_create.argtypes = [ctypes.POINTER(WlcParams), dp, ctypes.c_int,
                                    ctypes.POINTER(ScftOpts), ctypes.POINTER(vp)]
     L.wlc_batch_size.argtypes = [vp]
@@ -319,6 +321,15 @@ def make_params(shape, box, Nblk, Lmax, lam, spectrum_fp32=False):
         p.lam[b] = float(lam[b])
     p.spectrum_fp32 = 1 if spectrum_fp32 else 0
     return p
+
+
+def step_noalloc(params, direction, s, hist_ptrs, w_ptr, out_ptr, stream=None):
+    """wlc_mde_step_noalloc: one contour step with no handle and no allocation (the scratch
+    is the library's fixed device array), queued on `stream` (an integer hipStream_t)."""
+    load()
+    _check(_lib.wlc_mde_step_noalloc(ctypes.byref(params), direction, s, _ptr3(hist_ptrs),
+                                     ctypes.c_void_p(w_ptr), ctypes.c_void_p(out_ptr),
+                                     ctypes.c_void_p(stream)))
 
 
 def scft_opts(max_iter=500, n_simple=20, n_hist=10, form=0, lam_simple=0.05, lam_anderson=0.1,
