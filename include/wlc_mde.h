@@ -191,7 +191,13 @@ typedef struct wlc_scft_opts {
   double lam_simple;    /* simple-mixing step (lanbtWA / lanbtWB) */
   double lam_anderson;  /* Anderson step (lambda_WAB_anderson) */
   double tol;           /* converged when every species' residual is below tol */
-  int reserved[4];      /* must be zero */
+  int adaptive;         /* 1: the Anderson step grows 5 % per iteration that lowers the
+                           largest residual, up to lam_max; when that residual exceeds ten
+                           times the best since the last restart the history restarts from the
+                           current entry and the step halves (no counterpart in the reference,
+                           whose step is fixed) */
+  int reserved[3];      /* must be zero */
+  double lam_max;       /* adaptive mode: largest Anderson step (0: 1.0) */
 } wlc_scft_opts;
 
 typedef struct wlc_scft_status {
@@ -202,6 +208,8 @@ typedef struct wlc_scft_status {
   double F;             /* free energy per chain of the last iteration (kT) */
   double lnQ;
   double resid[3];      /* residual per species of the last iteration */
+  int restarts;         /* Anderson history restarts (adaptive mode) */
+  int pad2;
   double time_ms;       /* wall time from the first iteration to the end of the iteration in
                            which the instance finished (-1 while it runs) */
   double elapsed_ms;    /* wall time of all iterations so far */

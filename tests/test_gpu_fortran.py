@@ -49,8 +49,12 @@ class Mixer:
     def update(self, it, w, phi, chi, f):
         nb = len(w)
         xi = w.mean(0)
-        t = np.array([xi - w[a] + sum(chi[a][b] * (phi[b] - f[b]) for b in range(nb) if b != a)
-                      for a in range(nb)])
+        # incompressibility gain: half the mean pair chi (two blocks: the reference's force)
+        kappa = (sum(chi[a][b] for a in range(nb) for b in range(a + 1, nb)) / (nb * (nb - 1))
+                 if nb > 1 else 0.0)
+        s = [sum(chi[a][b] * (phi[b] - f[b]) for b in range(nb) if b != a) for a in range(nb)]
+        sbar = sum(s) / nb
+        t = np.array([s[a] - sbar + kappa * (phi.sum(0) - 1.0) + xi - w[a] for a in range(nb)])
         wo = w + t
         if it <= self.ns:
             if it == self.ns:

@@ -40,6 +40,10 @@ struct MixState {
   int done;       // converged or out of iterations; copied to the skip flag by commit
   int converged;
   int nr;         // Anderson coefficients in c
+  int nvalid;     // history entries usable by the next Anderson step (reset by a restart)
+  int restarts;
+  double lam;     // current Anderson step (adaptive mode), else the fixed one
+  double rbest;   // smallest largest-species residual of the Anderson phase so far
   double Qint, totden, lnQ, F, fe;  // Qint = V Q (the volume integral of q_0(Ns))
   double resid[3];
   double G[(kMaxHist + 1) * (kMaxHist + 1)];  // Gram matrix <dW_i, dW_j> of the history slots
@@ -49,6 +53,8 @@ struct MixState {
 struct MixOpts {
   int n_simple, n_hist, form, max_iter;
   double lam_s, lam_a, tol;
+  int adaptive;
+  double lam_max;
 };
 
 struct DeltaJob {
@@ -98,6 +104,7 @@ struct MixJob {
   double chi[9];
   double f[3];
   double V, dV;
+  double kappa;           // half the mean pair chi: gain of the incompressibility force
   double* w;              // [nblk] padded fields
   const double* phi;
   double* wout;           // [n_hist + 1][nblk] padded fields
@@ -228,8 +235,10 @@ __device__ __forceinline__ int hist_slot(int i, int nh) {
 }
 
 // Field update, pass 1 (SCFT_iterate.f90:140-333 generalised to n blocks): per point the
-// force t_a = sum_{b /= a} chi_ab (phi_b - f_b) + xi - w_a (xi = mean field), Wout_a = w_a +
-// t_a and dW_a = Wout_a - w_a into history slot k; simple mixing w_a += lam_s t_a.  Block
+// force t_a = s_a - mean_a s_a + kappa (sum_b phi_b - 1) + xi - w_a with s_a = sum_{b /= a}
+// chi_ab (phi_b - f_b), xi = mean field and kappa half the mean pair chi (for two blocks the
+// reference's force exactly; the common mode drives incompressibility), Wout_a = w_a + t_a
+// and dW_a = Wout_a - w_a into history slot k; simple mixing w_a += lam_s t_a.  Block
 // partial sums: field-energy integrand, <xi>, max |t_a|, <dW_a^2>, <Wout_a^2>, <Wout_a>
 // and the new row <dW_k, dW_(k - m)> of the Gram matrix (m = 0..nr).
 __global__ void __launch_bounds__(256) kb_mix_force(const MixJob* __restrict__ jobs, MixOpts o) {
@@ -241,7 +250,7 @@ __global__ void __launch_bounds__(256) kb_mix_force(const MixJob* __restrict__ j
   const bool simple = it <= o.n_simple;
   const bool write_hist = it >= o.n_simple;
   const int k = simple ? 0 : hist_slot(it - o.n_simple, nh);
-  const int nr = simple ? 0 : min(it - o.n_simple, nh);
+  const int nr = simple ? 0 : min(j.st->nvalid, nh);
   const long long hs = g.hs;
   double acc[kQ];
 #pragma unroll
@@ -272,10 +281,16 @@ __global__ void __launch_bounds__(256) kb_mix_force(const MixJob* __restrict__ j
     acc[0] += W * e;
     acc[1] += W * xi;
     double dwv[3] = {0.0, 0.0, 0.0};
+    double sa[3] = {0.0, 0.0, 0.0}, sbar = 0.0;
     for (int a = 0; a < nb; ++a) {
-      double ta = xi - w[a];
       for (int b = 0; b < nb; ++b)
-        if (b != a) ta += j.chi[a * 3 + b] * (ph[b] - j.f[b]);
+        if (b != a) sa[a] += j.chi[a * 3 + b] * (ph[b] - j.f[b]);
+      sbar += sa[a];
+    }
+    sbar /= nb;
+    const double inc = j.kappa * (sphi - 1.0);
+    for (int a = 0; a < nb; ++a) {
+      const double ta = sa[a] - sbar + inc + xi - w[a];
       const double wo = w[a] + ta;
       dwv[a] = wo - w[a];
       acc[3 + a] = fmax(acc[3 + a], fabs(ta));
@@ -359,7 +374,7 @@ __global__ void __launch_bounds__(64) kb_mix_solve(const MixJob* __restrict__ jo
   const int it = st->iter + 1;
   const bool simple = it <= o.n_simple;
   const int k = simple ? 0 : hist_slot(it - o.n_simple, nh);
-  const int nr = simple ? 0 : min(it - o.n_simple, nh);
+  const int nr = simple ? 0 : min(st->nvalid, nh);
   if (threadIdx.x < kQ) {
     const int q = threadIdx.x;
     const bool mx = q >= 3 && q <= 5;
@@ -382,7 +397,10 @@ __global__ void __launch_bounds__(64) kb_mix_solve(const MixJob* __restrict__ jo
   const int ld = kMaxHist + 1;
   if (simple) {
     for (int a = 0; a < nb; ++a) res[a] = tot[3 + a];
-    if (it == o.n_simple) G[0] = ddk / V;
+    if (it == o.n_simple) {
+      G[0] = ddk / V;
+      st->nvalid = 1;
+    }
   } else {
     double xbar = 0.0;
     for (int a = 0; a < nb; ++a)
@@ -412,14 +430,36 @@ __global__ void __launch_bounds__(64) kb_mix_solve(const MixJob* __restrict__ jo
     for (int m = 0; m < nr; ++m) st->c[m] = Vv[m];
   }
   st->nr = nr;
+  if (!simple) {
+    // history for the next step: one more entry, or (adaptive mode) a restart from the current
+    // entry alone with half the step when the largest residual has grown tenfold past the best
+    // since the last restart; the step grows 5 % per improving iteration up to lam_max
+    double rmax = 0.0;
+    for (int a = 0; a < nb; ++a) rmax = fmax(rmax, res[a]);
+    if (o.adaptive && rmax > 10.0 * st->rbest) {
+      st->nvalid = 1;
+      st->restarts += 1;
+      st->lam = fmax(0.5 * st->lam, 0.01);
+      st->nr = 0;  // this iteration's update: a plain step from the current entry
+      st->rbest = rmax;  // the best is measured again from the restart
+    } else {
+      st->nvalid = st->nvalid + 1;
+      if (o.adaptive && rmax < st->rbest) st->lam = fmin(o.lam_max, 1.05 * st->lam);
+      if (rmax < st->rbest) st->rbest = rmax;
+    }
+  }
   bool conv = true;
   for (int a = 0; a < 3; ++a) {
     st->resid[a] = a < nb ? res[a] : 0.0;
     if (a < nb && !(res[a] < o.tol)) conv = false;
   }
+  // a non-finite Q, free energy or residual (fields grown past what the explicit field term of
+  // the BDF step tolerates) ends the instance as failed
+  bool bad = !isfinite(st->lnQ) || !isfinite(st->F);
+  for (int a = 0; a < nb; ++a) bad = bad || !isfinite(res[a]);
   st->iter = it;
-  st->converged = conv ? 1 : 0;
-  st->done = (conv || it >= o.max_iter) ? 1 : 0;
+  st->converged = (conv && !bad) ? 1 : 0;
+  st->done = (conv || bad || it >= o.max_iter) ? 1 : 0;
 }
 
 // Field update, pass 3 (Anderson phase): w = sum_m c_m Wout_m + lam sum_m c_m dW_m (form 0,
@@ -452,7 +492,7 @@ __global__ void __launch_bounds__(256) kb_mix_combine(const MixJob* __restrict__
         di = di + st->c[m - 1] * (j.dw[((long long)km * nb + a) * hs + pr] - dk);
       }
       if (o.form == 1) wi = wi - di;
-      j.w[a * hs + pr] = wi + o.lam_a * di;
+      j.w[a * hs + pr] = wi + st->lam * di;
     }
   }
 }
@@ -835,6 +875,13 @@ void build_mix(const wlc_batch* b, JobTable& t, std::vector<Op>& ops) {
     for (int k = 0; k < 3; ++k) m.f[k] = I.f[k];
     m.V = I.V;
     m.dV = I.dV;
+    {
+      const int nb = I.p.nblk;
+      double k = 0.0;
+      for (int a = 0; a < nb; ++a)
+        for (int c = a + 1; c < nb; ++c) k += I.chi[a * 3 + c];
+      m.kappa = nb > 1 ? k / (nb * (nb - 1)) : 0.0;
+    }
     m.w = I.w;
     m.phi = I.phi;
     m.wout = I.wout;
@@ -980,6 +1027,7 @@ int wlc_batch_create(const wlc_params* params, const double* chi, int n, const w
   if (o.n_hist < 1 || o.n_hist > kMaxHist) return host_fail(WLC_EINVAL, "n_hist must be 1..16");
   if (o.n_simple < 0 || o.max_iter < 1 || (o.form != 0 && o.form != 1))
     return host_fail(WLC_EINVAL, "bad mixing options");
+  if (o.adaptive != 0 && o.adaptive != 1) return host_fail(WLC_EINVAL, "adaptive must be 0 or 1");
   for (int r : o.reserved)
     if (r) return host_fail(WLC_EINVAL, "reserved option fields must be zero");
   for (int i = 0; i < n; ++i) {
@@ -998,7 +1046,8 @@ int wlc_batch_create(const wlc_params* params, const double* chi, int n, const w
     delete b;
     return rc;
   }
-  b->mo = MixOpts{o.n_simple, o.n_hist, o.form, o.max_iter, o.lam_simple, o.lam_anderson, o.tol};
+  b->mo = MixOpts{o.n_simple, o.n_hist,   o.form, o.max_iter, o.lam_simple, o.lam_anderson,
+                  o.tol,      o.adaptive, o.lam_max > 0.0 ? o.lam_max : 1.0};
   if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
     free_batch(b);
     return host_fail(WLC_EHIP, "stream creation failed");
@@ -1108,7 +1157,13 @@ int wlc_batch_create(const wlc_params* params, const double* chi, int n, const w
   BALLOC(b->dstate, sizeof(MixState) * n);
   BALLOC(b->dskip, sizeof(int) * n);
 #undef BALLOC
-  if (hipMemset(b->dstate, 0, sizeof(MixState) * n) != hipSuccess ||
+  std::vector<MixState> init(n);
+  std::memset(init.data(), 0, sizeof(MixState) * n);
+  for (MixState& m : init) {
+    m.lam = o.lam_anderson;
+    m.rbest = 1e300;
+  }
+  if (hipMemcpy(b->dstate, init.data(), sizeof(MixState) * n, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(b->dskip, 0, sizeof(int) * n) != hipSuccess ||
       hipHostMalloc((void**)&b->hskip, sizeof(int) * n) != hipSuccess ||
       hipHostMalloc((void**)&b->hstate, sizeof(MixState) * n) != hipSuccess) {
@@ -1262,6 +1317,7 @@ int wlc_batch_status(wlc_batch* b, int i, wlc_scft_status* out) {
   out->F = s.F;
   out->lnQ = s.lnQ;
   for (int a = 0; a < 3; ++a) out->resid[a] = s.resid[a];
+  out->restarts = s.restarts;
   out->time_ms = b->done_ms[i];
   out->elapsed_ms = b->elapsed_ms;
   return WLC_OK;

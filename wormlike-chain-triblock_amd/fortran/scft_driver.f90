@@ -15,6 +15,9 @@
 !   --split-operator-slot  set up and solve through the split-operator call slot
 !            (split_operator_MDE_init / AB_diblock_sph_driver) instead of MDE_setup /
 !            MDE_q / MDE_qstar (same results: the slot runs the same solve)
+!   --adaptive <lam_max>  adaptive Anderson step (module scft_iterate: grows to lam_max while
+!            the residual falls, history restart and half step when it grows tenfold past the
+!            best since the last restart)
 !   --logs   write the reference's run logs in the working directory (created empty at
 !            start as initialize_mod.f90:296-306 does; SCFT_dump.f90:35-56 and :183-197
 !            append after every iteration): error.dat (iteration, residual per species),
@@ -40,8 +43,8 @@ program scft_driver
   implicit none
   character(len=1024) :: fin, fout, fw, arg
   integer :: u, hdr(16), max_iter, method, it, a, b, n, nargs, done, i, npos, stat
-  logical :: read_w, dump_w, sph_slot, fp32_on, next_is_tol, logs
-  real(8) :: fp32_tol
+  logical :: read_w, dump_w, sph_slot, fp32_on, next_is_tol, logs, next_is_lmax
+  real(8) :: fp32_tol, lmax_arg
   integer(8) :: c0, c1, rate
   real(8) :: box(3), lam(3), chi9(9), chi(3, 3), f(3), tol, mix(2)
   logical :: converged
@@ -50,6 +53,7 @@ program scft_driver
   fin = ''; fout = ''; fw = ''
   read_w = .false.; dump_w = .false.; sph_slot = .false.; logs = .false.
   fp32_on = .false.; next_is_tol = .false.; fp32_tol = 0d0
+  next_is_lmax = .false.; lmax_arg = 1d0
   npos = 0
   do i = 1, nargs
      call get_command_argument(i, arg)
@@ -57,6 +61,10 @@ program scft_driver
         read(arg, *) fp32_tol
         fp32_on = fp32_tol > 0d0
         next_is_tol = .false.
+     else if (next_is_lmax) then
+        read(arg, *) lmax_arg
+        adaptive = .true.
+        next_is_lmax = .false.
      else if (trim(arg) == '--spectrum-fp32') then
         next_is_tol = .true.
      else if (trim(arg) == '--readW') then
@@ -67,6 +75,8 @@ program scft_driver
         sph_slot = .true.
      else if (trim(arg) == '--logs') then
         logs = .true.
+     else if (trim(arg) == '--adaptive') then
+        next_is_lmax = .true.
      else
         npos = npos + 1
         if (npos == 1) fin = arg
@@ -74,9 +84,9 @@ program scft_driver
         if (npos == 3) fw = arg
      end if
   end do
-  if (npos < 2 .or. npos > 3 .or. next_is_tol) then
+  if (npos < 2 .or. npos > 3 .or. next_is_tol .or. next_is_lmax) then
      write(*, '(a)') 'usage: scft_driver <input.bin> <output.txt> [final_w.bin] [--readW] [--dumpW] &
-          &[--spectrum-fp32 <tol>] [--split-operator-slot] [--logs]'
+          &[--spectrum-fp32 <tol>] [--split-operator-slot] [--logs] [--adaptive <lam_max>]'
      error stop 2
   end if
 
@@ -90,6 +100,7 @@ program scft_driver
   if (method == 0) n_simple = max_iter + 1
   read(u) box, lam, chi9, mix, tol
   lam_simple = mix(1); lam_anderson = mix(2)
+  lam_max = lmax_arg
   box_x = box(1); box_y = box(2); box_z = box(3)
   lambda_blk = lam
   do a = 1, 3

@@ -58,8 +58,9 @@ def write_input(path, shape, box, Nblk, L, lam, chi, w, max_iter, method, n_simp
 
 
 def run_case(method=0, max_iter=50, n_simple=20, n_hist=10, lam_s=0.05, lam_a=0.1, tol=1e-6,
-             form=0, case=None, timeout=1200):
-    """Runs the driver; returns (rows, info): rows[i] = (iter, F, lnQ, max residual,
+             form=0, case=None, timeout=1200, adaptive_lam_max=None):
+    """Runs the driver (adaptive_lam_max: the adaptive Anderson step up to that value);
+    returns (rows, info): rows[i] = (iter, F, lnQ, max residual,
     residual A, B, C, mean phi A, B, C, max |w|); info = dict(iterations, converged,
     seconds, error).  A driver that stops on an error (e.g. non-finite Q once the fields
     outgrow the explicit field term of the BDF step) returns the iterations it completed
@@ -71,7 +72,8 @@ def run_case(method=0, max_iter=50, n_simple=20, n_hist=10, lam_s=0.05, lam_a=0.
         inp, out = os.path.join(d, "in.bin"), os.path.join(d, "out.txt")
         write_input(inp, c["shape"], c["box"], c["Nblk"], c["L"], c["lam"], c["chi"], c["w"],
                     max_iter, method, n_simple, n_hist, lam_s, lam_a, tol, form)
-        r = subprocess.run([DRIVER, inp, out], capture_output=True, text=True, timeout=timeout)
+        flags = [] if adaptive_lam_max is None else ["--adaptive", repr(float(adaptive_lam_max))]
+        r = subprocess.run([DRIVER, inp, out] + flags, capture_output=True, text=True, timeout=timeout)
         lines = open(out).read().splitlines() if os.path.exists(out) else []
     rows = np.array([[float(x) for x in ln.split()] for ln in lines if not ln.startswith("#")])
     tail = [ln for ln in lines if ln.startswith("#")]

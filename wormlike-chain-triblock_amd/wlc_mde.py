@@ -49,7 +49,7 @@ class ScftOpts(ctypes.Structure):
         ("max_iter", ctypes.c_int), ("n_simple", ctypes.c_int), ("n_hist", ctypes.c_int),
         ("form", ctypes.c_int), ("lam_simple", ctypes.c_double),
         ("lam_anderson", ctypes.c_double), ("tol", ctypes.c_double),
-        ("reserved", ctypes.c_int * 4),
+        ("adaptive", ctypes.c_int), ("reserved", ctypes.c_int * 3), ("lam_max", ctypes.c_double),
     ]
 
 
@@ -57,7 +57,8 @@ class ScftStatus(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("done", ctypes.c_int),
         ("pad", ctypes.c_int), ("F", ctypes.c_double), ("lnQ", ctypes.c_double),
-        ("resid", ctypes.c_double * 3), ("time_ms", ctypes.c_double),
+        ("resid", ctypes.c_double * 3), ("restarts", ctypes.c_int), ("pad2", ctypes.c_int),
+        ("time_ms", ctypes.c_double),
         ("elapsed_ms", ctypes.c_double),
     ]
 
@@ -333,10 +334,11 @@ def step_noalloc(params, direction, s, hist_ptrs, w_ptr, out_ptr, stream=None):
 
 
 def scft_opts(max_iter=500, n_simple=20, n_hist=10, form=0, lam_simple=0.05, lam_anderson=0.1,
-              tol=1e-6):
+              tol=1e-6, adaptive=False, lam_max=1.0):
     o = ScftOpts()
     o.max_iter, o.n_simple, o.n_hist, o.form = max_iter, n_simple, n_hist, form
     o.lam_simple, o.lam_anderson, o.tol = lam_simple, lam_anderson, tol
+    o.adaptive, o.lam_max = 1 if adaptive else 0, lam_max
     return o
 
 
@@ -412,7 +414,8 @@ class Batch:
         st = ScftStatus()
         _check(_lib.wlc_batch_status(self.h, i, ctypes.byref(st)))
         return dict(iterations=st.iterations, converged=bool(st.converged), done=bool(st.done),
-                    F=st.F, lnQ=st.lnQ, resid=list(st.resid), time_ms=st.time_ms,
+                    F=st.F, lnQ=st.lnQ, resid=list(st.resid), restarts=st.restarts,
+                    time_ms=st.time_ms,
                     elapsed_ms=st.elapsed_ms)
 
 
