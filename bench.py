@@ -237,9 +237,16 @@ def bench_spectrum_fp32(W, args):
     }))
 
 
-def c4_cases(n):
+# field update of both time-to-residual runs (plain and batched): 20 simple-mixing iterations
+# (step 0.05), then Anderson over 10 histories mixing the inputs with step 0.1 (the reference's
+# form, mixing the outputs, diverges on most of these instances)
+MIX = dict(n_simple=20, n_hist=10, lam_s=0.05, lam_a=0.1, form=1)
+
+
+def c4_cases(args):
     import batch_case as B
-    return B.draw()[:n]
+    lo, hi = (int(x) for x in args.c4_range.split(":")) if args.c4_range else (0, args.c4)
+    return list(enumerate(B.draw()))[lo:hi]
 
 
 def c4_report(kind, rows, extra):
@@ -249,18 +256,19 @@ def c4_report(kind, rows, extra):
     conv = [r["converged"] for r in rows]
     tc = t[np.array(conv, dtype=bool)] if any(conv) else np.array([np.nan])
     out = {
-        "metric": f"time-to-residual 1e-6, {kind}",
+        "metric": f"time-to-residual {extra['tol']:g}, {kind}",
         "value": round(float(np.mean(t)), 4), "unit": "s", "higher_is_better": False,
         "n_gpus": 1, "dtype": "f64", "data": "synthetic",
         "instances": len(rows), "converged": int(sum(conv)),
         "mean_s": round(float(np.mean(t)), 4), "median_s": round(float(np.median(t)), 4),
         "mean_s_converged": round(float(np.mean(tc)), 4),
         "mean_iterations": round(float(np.mean(it)), 1), "median_iterations": float(np.median(it)),
-        "config": {"workload": "48 linear ABC wormlike triblocks from seed 3: cubic grid edge "
-                               "{24,32,48,64}, Lmax {4,6}, f_A, f_B ~ U[0.2,0.45], chiN_ij ~ U[12,30], "
-                               "lambda ~ U[0.25,4], box ~ U[3,6], Ns 100, w = 0.1 N(0,1); 20 "
-                               "simple-mixing iterations (0.05) then Anderson (10 histories, 0.1), "
-                               "at most 520 iterations (500 Anderson), residual 1e-6",
+        "config": {"workload": "linear ABC wormlike triblocks from seed 3 (the 48-instance batching "
+                               "draw): cubic grid edge {24,32,48,64}, Lmax {4,6}, f_A, f_B ~ "
+                               "U[0.2,0.45], chiN_ij ~ U[12,30], lambda ~ U[0.25,4], box ~ U[3,6], "
+                               "Ns 100, w = 0.1 N(0,1); 20 simple-mixing iterations (0.05) then "
+                               "Anderson (10 histories, inputs mixed, 0.1), at most 520 iterations "
+                               "(500 Anderson)", "tol": extra.pop("tol"),
                    "mixing": "simple then Anderson"},
         "per_instance": [{k: (round(v, 10) if isinstance(v, float) else v) for k, v in r.items()}
                          for r in rows],
@@ -274,14 +282,32 @@ def bench_plain_c4(args):
     field update on the host), one after the other; time = the driver's loop wall time."""
     import scft_case as S
     rows = []
-    for i, c in enumerate(c4_cases(args.c4)):
-        r, info = S.run_case(method=1, max_iter=520, n_simple=20, n_hist=10, lam_s=0.05, lam_a=0.1,
-                             tol=1e-6, case=c, timeout=900)
+    for i, c in c4_cases(args):
+        r, info = S.run_case(method=1, max_iter=520, tol=args.tol, case=c, timeout=900, **MIX)
         rows.append({"i": i, "edge": c["shape"][0], "L": c["L"][0], "iterations": info["iterations"],
                      "converged": bool(info["converged"]), "time_s": info["seconds"],
                      "F": float(r[-1, 1]) if len(r) else None, "error": info["error"]})
         print(f"# plain {i}: {rows[-1]}", file=sys.stderr, flush=True)
-    c4_report("plain single-instance SCFT (Fortran driver, one instance at a time)", rows, {})
+    c4_report("plain single-instance SCFT (Fortran driver, one instance at a time)", rows,
+              {"tol": args.tol})
+
+
+def bench_warm_train(args, W):
+    """Training data from the batched solver (draws of the batching workload's generator at
+    grid edges 24 and 32, Lmax 4 or 6, seed 11), then training of the warm-start network on
+    them; writes the weights to --out."""
+    import warmstart as WS
+    log = lambda m: print("# " + m, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    data, info = WS.generate(W, args.warm_instances, 11, (24, 32), (4, 6), chunk=64, log=log)
+    t_gen = time.perf_counter() - t0
+    net, hist, t_train = WS.train(W, data, args.warm_epochs, batch=4, lr=1e-3, seed=0, log=log)
+    np.save(args.out, net.params())
+    print(json.dumps({"metric": "warm-start network training", "samples": {int(e): len(X) for e, (X, _) in data.items()},
+                      "instances": len(info), "generate_s": round(t_gen, 1), "train_s": round(t_train, 1),
+                      "epochs": args.warm_epochs, "loss": [round(float(h), 6) for h in hist],
+                      "params": net.num_params(), "weights": args.out}), flush=True)
+    net.close()
 
 
 def bench_batch_c4(args, W):
@@ -289,26 +315,51 @@ def bench_batch_c4(args, W):
     for every instance, field update on the device, one SCFT iteration replayed from a HIP
     graph); time of an instance = wall time from the first iteration to the end of the
     iteration in which it converged (or stopped)."""
-    cases = c4_cases(args.c4)
-    b = W.Batch(cases, W.scft_opts(max_iter=520, n_simple=20, n_hist=10, lam_simple=0.05,
-                                   lam_anderson=0.1, tol=1e-6))
+    idx_cases = c4_cases(args)
+    cases = [c for _, c in idx_cases]
+    b = W.Batch(cases, W.scft_opts(max_iter=520, n_simple=MIX["n_simple"], n_hist=MIX["n_hist"],
+                                   lam_simple=MIX["lam_s"], lam_anderson=MIX["lam_a"],
+                                   form=MIX["form"], tol=args.tol))
+    t_net = 0.0
+    init = [c["w"] for c in cases]
+    if args.warm:
+        import warmstart as WS
+        net = W.UNet(cases[0]["shape"][0], 1)
+        net.set_params(np.load(args.warm))
+        t1 = time.perf_counter()
+        init = WS.predict(net, cases)
+        t_net = time.perf_counter() - t1
+        net.close()
     for i, c in enumerate(cases):
-        b.set_w(i, c["w"])
+        b.set_w(i, init[i])
+    print(f"# batch of {len(cases)} instances created, {b.launches(True)} launches per iteration"
+          + (f", warm start from {args.warm} ({t_net:.2f} s)" if args.warm else ""),
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    while b.iterate(20, use_graph=True) > 0:
-        pass
+    done = 0
+    while True:
+        active = b.iterate(10, use_graph=True)
+        done += 10
+        print(f"# {done} iterations, {active} instances advancing, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        if active == 0:
+            break
     wall = time.perf_counter() - t0
     rows = []
-    for i, c in enumerate(cases):
-        st = b.status(i)
+    for k, (i, c) in enumerate(idx_cases):
+        st = b.status(k)
         rows.append({"i": i, "edge": c["shape"][0], "L": c["L"][0], "iterations": st["iterations"],
-                     "converged": st["converged"], "time_s": st["time_ms"] / 1e3, "F": st["F"],
-                     "resid": max(st["resid"])})
+                     "converged": st["converged"], "time_s": st["time_ms"] / 1e3 + t_net,
+                     "F": st["F"], "resid": max(st["resid"])})
     launches = b.launches(True)
     b.close()
-    c4_report("batched SCFT (all instances per launch, device field update, HIP graph)", rows,
-              {"batch_wall_s": round(wall, 3), "amortised_s_per_instance": round(wall / len(rows), 4),
-               "launches_per_iteration": launches})
+    kind = "batched SCFT (all instances per launch, device field update, HIP graph)"
+    if args.warm:
+        kind = "batched warm-started SCFT (network proposal as initial fields; " + kind[15:]
+    c4_report(kind, rows,
+              {"batch_wall_s": round(wall + t_net, 3),
+               "amortised_s_per_instance": round((wall + t_net) / len(rows), 4),
+               "network_s": round(t_net, 3), "launches_per_iteration": launches, "tol": args.tol})
 
 
 def main():
@@ -339,6 +390,15 @@ def main():
     ap.add_argument("--batch-c4", action="store_true",
                     help="time-to-residual of the same instances advanced together in one batch")
     ap.add_argument("--c4", type=int, default=48, help="instances of the batching workload")
+    ap.add_argument("--c4-range", default="", help="lo:hi slice of the batching workload's instances")
+    ap.add_argument("--tol", type=float, default=1e-6, help="residual of the time-to-residual runs")
+    ap.add_argument("--warm", default="", help="weights (.npy) of the warm-start network for --batch-c4")
+    ap.add_argument("--warm-train", action="store_true",
+                    help="generate training data with the batched solver and train the warm-start "
+                         "network; weights to --out")
+    ap.add_argument("--warm-instances", type=int, default=256)
+    ap.add_argument("--warm-epochs", type=int, default=30)
+    ap.add_argument("--out", default="unet_weights.npy")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the counter "
                          "calibration kernel of wormlike-chain-triblock_amd/profile_summary.py "
@@ -398,6 +458,9 @@ def main():
 
     if args.batch_c4:
         bench_batch_c4(args, W)
+        return
+    if args.warm_train:
+        bench_warm_train(args, W)
         return
     if args.grid128:
         bench_grid128(W, args)

@@ -11,10 +11,10 @@ and parameters to proposed initial fields; plain SCFT from the seed fields stays
                       so it rescales Q and leaves phi and F unchanged), / 10
   proposal(y)         fields = 10 y, used as the initial w of SCFT
 
-generate(...) runs the batching workload's generator at a fixed grid and Lmax through the
-batched solver and keeps the converged instances; train(...) fits the network; evaluate(...)
-runs held-out draws from their seed fields and from the network's proposal through the same
-batched solver and reports iterations, time and the free-energy difference.
+generate(...) runs the batching workload's generator at small grids through the batched solver
+and keeps the final fields of the instances that converged or came close; train(...) fits the
+network; predict(...) proposes initial fields for new instances at any grid edge that is a
+multiple of 8 (the network is fully convolutional).
 """
 import time
 
@@ -25,11 +25,14 @@ import batch_case as B
 SCALE = 10.0
 
 
-def draw_at(n_inst, seed, edge, L, Ns=B.NS):
-    """The batching workload's parameter draws with the grid edge and Lmax fixed."""
+def draw_at(n_inst, seed, edges, Ls, Ns=B.NS):
+    """The batching workload's parameter draws with the grid edge and Lmax drawn from the given
+    sets (training data at small grids)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n_inst):
+        edge = int(edges[rng.integers(len(edges))])
+        L = int(Ls[rng.integers(len(Ls))])
         fA = float(rng.uniform(0.2, 0.45))
         fB = float(rng.uniform(0.2, 0.45))
         fC = 1.0 - fA - fB
@@ -66,15 +69,21 @@ def proposal(y):
     return SCALE * np.moveaxis(np.asarray(y, dtype=np.float64), -1, 0)
 
 
-def solve(W, cases, init=None, max_iter=520, graph=True):
-    """Batched SCFT of cases from their seed fields (or init[i]); per-instance results."""
+def solve(W, cases, init=None, max_iter=520, tol=1e-4, graph=True, log=print):
+    """Batched SCFT of cases from their seed fields (or init[i]): 20 simple-mixing iterations
+    (0.05), then Anderson over 10 histories mixing the inputs (0.1); per-instance results."""
     b = W.Batch(cases, W.scft_opts(max_iter=max_iter, n_simple=20, n_hist=10, lam_simple=0.05,
-                                   lam_anderson=0.1, tol=1e-6))
+                                   lam_anderson=0.1, form=1, tol=tol))
     for i, c in enumerate(cases):
         b.set_w(i, c["w"] if init is None else init[i])
     t0 = time.perf_counter()
-    while b.iterate(20, use_graph=graph) > 0:
-        pass
+    it = 0
+    while True:
+        act = b.iterate(20, use_graph=graph)
+        it += 20
+        log(f"solve: {it} iterations, {act} of {len(cases)} advancing, {time.perf_counter() - t0:.1f} s")
+        if act == 0:
+            break
     wall = time.perf_counter() - t0
     res = []
     for i in range(len(cases)):
@@ -85,44 +94,56 @@ def solve(W, cases, init=None, max_iter=520, graph=True):
     return res, wall
 
 
-def generate(W, n_inst, seed, edge, L, chunk=64, log=print):
-    """Converged (features, target) pairs from batched SCFT of draw_at(...) instances."""
-    X, T, info = [], [], []
-    cases = draw_at(n_inst, seed, edge, L)
+def generate(W, n_inst, seed, edges, Ls, chunk=64, max_iter=520, keep_below=1e-3, log=print):
+    """(features, target) pairs from batched SCFT of draw_at(...) instances: the final fields of
+    every instance whose largest residual fell below keep_below (converged to 1e-4 or close to
+    it after max_iter iterations), grouped by grid edge."""
+    data = {}
+    info = []
+    cases = draw_at(n_inst, seed, edges, Ls)
     for k in range(0, n_inst, chunk):
         part = cases[k:k + chunk]
-        res, wall = solve(W, part)
-        nconv = 0
+        res, wall = solve(W, part, max_iter=max_iter, log=log)
+        kept = 0
         for c, r in zip(part, res):
-            if r["converged"]:
+            ok = np.isfinite(r["F"]) and max(r["resid"]) < keep_below
+            if ok:
+                X, T = data.setdefault(c["shape"][0], ([], []))
                 X.append(features(c))
                 T.append(target(r["w"]))
-                nconv += 1
-            info.append(dict(iterations=r["iterations"], converged=r["converged"], F=r["F"]))
-        log(f"generate: instances {k}..{k + len(part) - 1}: {nconv} converged, {wall:.1f} s")
-    return np.stack(X), np.stack(T), info
+                kept += 1
+            info.append(dict(edge=c["shape"][0], L=c["L"][0], iterations=r["iterations"],
+                             converged=r["converged"], resid=max(r["resid"]), kept=bool(ok)))
+        log(f"generate: instances {k}..{k + len(part) - 1}: {kept} kept, {wall:.1f} s")
+    return {e: (np.stack(X), np.stack(T)) for e, (X, T) in data.items()}, info
 
 
-def train(W, X, T, epochs, batch=4, lr=1e-3, seed=0, log=print, net=None):
-    n = X.shape[1]
+def train(W, data, epochs, batch=4, lr=1e-3, seed=0, log=print, net=None):
+    """Adam with a cosine-decayed rate over mini-batches of one grid size at a time (the network
+    is fully convolutional; it is resized between sizes).  data: {edge: (X, T)}."""
+    first = min(data)
     if net is None:
-        net = W.UNet(n, batch, seed=seed)
-    else:
-        net.resize(n, batch)
+        net = W.UNet(first, batch, seed=seed)
     rng = np.random.default_rng(seed)
-    N = len(X)
     hist = []
     t0 = time.perf_counter()
+    nsamp = sum(len(X) for X, _ in data.values())
     for ep in range(epochs):
-        order = rng.permutation(N)
         rate = lr * (0.5 * (1 + np.cos(np.pi * ep / epochs)))
+        batches = []
+        for e in rng.permutation(sorted(data)):  # sizes in random order, each size's batches together
+            X = data[int(e)][0]
+            order = rng.permutation(len(X))
+            batches += [(int(e), order[k:k + batch]) for k in range(0, len(X) - batch + 1, batch)]
         tot = 0.0
-        for k in range(0, N - batch + 1, batch):
-            idx = order[k:k + batch]
+        for e, idx in batches:
+            if net.n != e or net.B != batch:
+                net.resize(e, batch)
+            X, T = data[e]
             tot += net.train_step(X[idx], T[idx], rate)
-        hist.append(tot / max(1, N // batch))
-        if ep % 10 == 0 or ep == epochs - 1:
-            log(f"train: epoch {ep} loss {hist[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+        hist.append(tot / max(1, len(batches)))
+        if ep % 5 == 0 or ep == epochs - 1:
+            log(f"train: epoch {ep} loss {hist[-1]:.5f} ({nsamp} samples, {time.perf_counter() - t0:.1f} s)")
     return net, hist, time.perf_counter() - t0
 
 
