@@ -246,7 +246,11 @@ MIX = dict(n_simple=20, n_hist=10, lam_s=0.05, lam_a=0.1, form=1)
 def c4_cases(args):
     import batch_case as B
     lo, hi = (int(x) for x in args.c4_range.split(":")) if args.c4_range else (0, args.c4)
-    return list(enumerate(B.draw()))[lo:hi]
+    out = list(enumerate(B.draw()))[lo:hi]
+    if args.c4_edges:
+        keep = {int(e) for e in args.c4_edges.split(",")}
+        out = [(i, c) for i, c in out if c["shape"][0] in keep]
+    return out
 
 
 def c4_report(kind, rows, extra):
@@ -267,8 +271,9 @@ def c4_report(kind, rows, extra):
                                "draw): cubic grid edge {24,32,48,64}, Lmax {4,6}, f_A, f_B ~ "
                                "U[0.2,0.45], chiN_ij ~ U[12,30], lambda ~ U[0.25,4], box ~ U[3,6], "
                                "Ns 100, w = 0.1 N(0,1); 20 simple-mixing iterations (0.05) then "
-                               "Anderson (10 histories, inputs mixed, 0.1), at most 520 iterations "
-                               "(500 Anderson)", "tol": extra.pop("tol"),
+                               "Anderson (10 histories, inputs mixed, 0.1), at most max_iter "
+                               "iterations (520: 500 Anderson)", "tol": extra.pop("tol"),
+                   "max_iter": extra.pop("max_iter"),
                    "mixing": "simple then Anderson"},
         "per_instance": [{k: (round(v, 10) if isinstance(v, float) else v) for k, v in r.items()}
                          for r in rows],
@@ -283,13 +288,13 @@ def bench_plain_c4(args):
     import scft_case as S
     rows = []
     for i, c in c4_cases(args):
-        r, info = S.run_case(method=1, max_iter=520, tol=args.tol, case=c, timeout=900, **MIX)
+        r, info = S.run_case(method=1, max_iter=args.max_iter, tol=args.tol, case=c, timeout=900, **MIX)
         rows.append({"i": i, "edge": c["shape"][0], "L": c["L"][0], "iterations": info["iterations"],
                      "converged": bool(info["converged"]), "time_s": info["seconds"],
                      "F": float(r[-1, 1]) if len(r) else None, "error": info["error"]})
         print(f"# plain {i}: {rows[-1]}", file=sys.stderr, flush=True)
     c4_report("plain single-instance SCFT (Fortran driver, one instance at a time)", rows,
-              {"tol": args.tol})
+              {"tol": args.tol, "max_iter": args.max_iter})
 
 
 def bench_warm_train(args, W):
@@ -317,7 +322,7 @@ def bench_batch_c4(args, W):
     iteration in which it converged (or stopped)."""
     idx_cases = c4_cases(args)
     cases = [c for _, c in idx_cases]
-    b = W.Batch(cases, W.scft_opts(max_iter=520, n_simple=MIX["n_simple"], n_hist=MIX["n_hist"],
+    b = W.Batch(cases, W.scft_opts(max_iter=args.max_iter, n_simple=MIX["n_simple"], n_hist=MIX["n_hist"],
                                    lam_simple=MIX["lam_s"], lam_anderson=MIX["lam_a"],
                                    form=MIX["form"], tol=args.tol))
     t_net = 0.0
@@ -359,7 +364,8 @@ def bench_batch_c4(args, W):
     c4_report(kind, rows,
               {"batch_wall_s": round(wall + t_net, 3),
                "amortised_s_per_instance": round((wall + t_net) / len(rows), 4),
-               "network_s": round(t_net, 3), "launches_per_iteration": launches, "tol": args.tol})
+               "network_s": round(t_net, 3), "launches_per_iteration": launches, "tol": args.tol,
+               "max_iter": args.max_iter})
 
 
 def main():
@@ -392,6 +398,8 @@ def main():
     ap.add_argument("--c4", type=int, default=48, help="instances of the batching workload")
     ap.add_argument("--c4-range", default="", help="lo:hi slice of the batching workload's instances")
     ap.add_argument("--tol", type=float, default=1e-6, help="residual of the time-to-residual runs")
+    ap.add_argument("--max-iter", type=int, default=520, help="iteration cap of the time-to-residual runs")
+    ap.add_argument("--c4-edges", default="", help="only instances with these grid edges (e.g. 24,32)")
     ap.add_argument("--warm", default="", help="weights (.npy) of the warm-start network for --batch-c4")
     ap.add_argument("--warm-train", action="store_true",
                     help="generate training data with the batched solver and train the warm-start "
