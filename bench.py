@@ -423,8 +423,8 @@ def main():
         rows, info0 = S.run_case(method=0, max_iter=50, lam_s=0.05)
         if info0["error"]:
             raise RuntimeError(info0["error"])
-        rows1, info1 = S.run_case(method=1, max_iter=1500, n_simple=20, n_hist=10, lam_s=0.05,
-                                  lam_a=0.6, tol=1e-6, form=1)
+        rows1, info1 = S.run_case(method=1, max_iter=1500, n_simple=MIX["n_simple"], n_hist=MIX["n_hist"],
+                                  lam_s=MIX["lam_s"], lam_a=MIX["lam_a"], tol=1e-6, form=MIX["form"])
         done = len(rows1)
         print(json.dumps({
             "metric": "time-to-residual 1e-6, plain SCFT (32^3 ABC wormlike triblock, Lmax=4, Ns=100)",
@@ -438,7 +438,7 @@ def main():
             "config": {"workload": "32^3, box 4.0, f=1/3 each, chiN=20 all pairs, lambda=1, "
                                    "Lmax=4, Ns=100 (33/33/34), w=0.1 N(0,1) seed 0 +/-2cos(2 pi z/box) "
                                    "on A/C; 20 simple-mixing iterations (step 0.05) then Anderson "
-                                   "(10 histories, inputs mixed, lambda 0.6), at most 1500 iterations",
+                                   "(10 histories, inputs mixed, lambda 0.1), at most 1500 iterations",
                        "mixing": "simple then Anderson"},
             "simple_mixing_50": {"F": rows[:, 1].round(10).tolist(),
                                  "lnQ": rows[:, 2].round(10).tolist(),
