@@ -297,6 +297,54 @@ def bench_plain_c4(args):
               {"tol": args.tol, "max_iter": args.max_iter})
 
 
+def unet_flops(layers, n, B):
+    """Algorithmic FLOPs of one forward pass of the warm-start network (real channels, not the
+    16-channel padding): 2 * voxels * 27 * Cin * Cout per convolution."""
+    tot = 0
+    for l in layers:
+        vox = B * (n >> l["level"]) ** 3
+        tot += 2 * vox * 27 * l["cin_real"] * l["cout_real"]
+    return tot
+
+
+def bench_unet(args, W):
+    """Training and inference throughput of the warm-start network at edge 64 (one instance
+    per batch, the size of the largest batching instances): wall time per train step (forward,
+    loss, backward, Adam) and per forward pass, host copies of inputs and outputs included;
+    FLOPs counted on the real channels (a training step: forward, input gradient except for
+    the first layer, weight gradient)."""
+    n, B = args.unet_edge, 1
+    net = W.UNet(n, B, seed=0)
+    layers = net.layers()
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((B, n, n, n, 11)).astype(np.float32)
+    t = 0.1 * rng.standard_normal((B, n, n, n, 3)).astype(np.float32)
+    for _ in range(args.warmup):
+        net.train_step(x, t, 1e-4)
+        net.forward(x)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.train_step(x, t, 1e-4)
+    t_train = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.forward(x)
+    t_fwd = (time.perf_counter() - t0) / args.steps
+    f_fwd = unet_flops(layers, n, B)
+    l0 = layers[0]
+    f_train = 3 * f_fwd - 2 * B * n ** 3 * 27 * l0["cin_real"] * l0["cout_real"]
+    params = net.num_params()
+    net.close()
+    print(json.dumps({
+        "metric": "warm-start network throughput", "unit": "TFLOP/s", "dtype": "bf16 (fp32 accumulate)",
+        "config": {"workload": f"U-Net, edge {n}, batch {B}, 11 input / 3 output channels, "
+                               f"{len(layers)} 3x3x3 convolutions", "params": params},
+        "forward_ms": round(1e3 * t_fwd, 3), "train_step_ms": round(1e3 * t_train, 3),
+        "forward_tflops": round(f_fwd / t_fwd / 1e12, 3), "train_tflops": round(f_train / t_train / 1e12, 3),
+        "forward_flop": f_fwd, "train_flop": f_train, "peak_tflops_bf16_dense": 2500.0,
+        "steps": args.steps, "warmup": args.warmup}), flush=True)
+
+
 def bench_warm_train(args, W):
     """Training data from the batched solver (draws of the batching workload's generator at
     grid edges 24 and 32, Lmax 4 or 6, seed 11), then training of the warm-start network on
@@ -407,6 +455,10 @@ def main():
     ap.add_argument("--warm-instances", type=int, default=256)
     ap.add_argument("--warm-epochs", type=int, default=30)
     ap.add_argument("--out", default="unet_weights.npy")
+    ap.add_argument("--unet", action="store_true",
+                    help="instead of the sweep benchmark: training and inference throughput of the "
+                         "warm-start network at --unet-edge (one JSON line)")
+    ap.add_argument("--unet-edge", type=int, default=64)
     ap.add_argument("--copy-only", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the counter "
                          "calibration kernel of wormlike-chain-triblock_amd/profile_summary.py "
@@ -469,6 +521,9 @@ def main():
         return
     if args.warm_train:
         bench_warm_train(args, W)
+        return
+    if args.unet:
+        bench_unet(args, W)
         return
     if args.grid128:
         bench_grid128(W, args)

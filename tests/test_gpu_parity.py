@@ -191,10 +191,11 @@ def test_fused_inverse_rhs_is_bit_identical(wlc, case, monkeypatch):
 
 
 @pytest.mark.parametrize("shape,L", [((64, 6, 8), (5, 7, 6)), ((128, 4, 8), (5, 7, 6)),
+                                     ((128, 4, 6), (8, 6, 8)),
                                      ((24, 6, 12), (4, 5, 7)), ((32, 4, 6), (6, 4, 4))])
 def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
-    """Lmax 5..7 at nz = 64 and 128 runs the z solve with the harmonics of a column split
-    over two waves (per z half at nz = 128); the unsplit kernel computes the same step
+    """Lmax 5..7 at nz = 64 and 5..8 at nz = 128 runs the z solve with the harmonics of a
+    column split over two waves (per z half at nz = 128); the unsplit kernel computes the same step
     (equal up to rounding: the split kernel passes k = 0 through identity-angle rotations
     instead of a diagonal shortcut).  nx/2 + 1 = 5 leaves padding columns in the last tile.
     At nz = 64 the whole-tile split kernel (the default) and the two-column split kernel

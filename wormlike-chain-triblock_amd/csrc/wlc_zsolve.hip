@@ -105,6 +105,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// wave_sync that also keeps the instruction scheduler from moving work across it (SB):
+// with all 81 harmonics of Lmax 8 in registers, scheduling freedom across the transform
+// steps costs spills
+template <bool SB>
+__device__ __forceinline__ void wave_sync_sb() {
+  wave_sync();
+  if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
+}
+
 // Workgroup = one kx tile (4 columns) of one ky row; wave w owns column kx = 4 t + w.
 // Per batch of 8 harmonics the four waves cooperatively load each harmonic's 4 KB tile
 // (64 z x 4 kx, contiguous) with 1 KB-coalesced loads into LDS, wave w's region holding
@@ -259,7 +268,7 @@ constexpr int kXch = (2 * kLS + 1) * 64;                       // crossing chain
 constexpr int kRegionS = kRegion > kXch + 2 ? kRegion : kXch + 2;  // per-wave LDS region
 
 // forward z transform of the NH harmonics in v (lane = z -> lane = kz), batches of 8
-template <int NH, int N>
+template <int NH, bool SB = false, int N>
 __device__ __forceinline__ void z64_fwd_regs(double2 (&v)[N], double2* wl, int lane,
                                              const double2 (&tw)[8]) {
   const int h = lane >> 3, j = lane & 7;
@@ -268,33 +277,33 @@ __device__ __forceinline__ void z64_fwd_regs(double2 (&v)[N], double2* wl, int l
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (b0 + q < NH) wl[q * kPadBatch + pad8(lane)] = v[b0 + q];
-    wave_sync();
+    wave_sync_sb<SB>();
     double2 u[8];
 #pragma unroll
     for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
     Dft<8, -1>::run(u);
 #pragma unroll
     for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<-1>(u[k2], tw[k2]);
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
     Dft<8, -1>::run(u);
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (b0 + q < NH) v[b0 + q] = wl[q * kPadBatch + pad8(lane)];
-    wave_sync();
+    wave_sync_sb<SB>();
   }
 }
 
 // inverse z transform (lane = kz -> lane = z), batches of 8
-template <int NH, int N>
+template <int NH, bool SB = false, int N>
 __device__ __forceinline__ void z64_inv_regs(double2 (&v)[N], double2* wl, int lane,
                                              const double2 (&tw)[8]) {
   const int h = lane >> 3, j = lane & 7;
@@ -303,28 +312,28 @@ __device__ __forceinline__ void z64_inv_regs(double2 (&v)[N], double2* wl, int l
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (b0 + q < NH) wl[q * kPadBatch + pad8(lane)] = v[b0 + q];
-    wave_sync();
+    wave_sync_sb<SB>();
     double2 u[8];
 #pragma unroll
     for (int n2 = 0; n2 < 8; ++n2) u[n2] = wl[h * kPadBatch + pad8(j + 8 * n2)];
     Dft<8, +1>::run(u);
 #pragma unroll
     for (int k2 = 1; k2 < 8; ++k2) u[k2] = twiddle<+1>(u[k2], tw[k2]);
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) wl[h * kPadBatch + pad8(8 * k2 + j)] = u[k2];
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) u[jj] = wl[h * kPadBatch + pad8(8 * j + jj)];
     Dft<8, +1>::run(u);
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) wl[h * kPadBatch + pad8(j + 8 * k1)] = u[k1];
-    wave_sync();
+    wave_sync_sb<SB>();
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (b0 + q < NH) v[b0 + q] = wl[q * kPadBatch + pad8(lane)];
-    wave_sync();
+    wave_sync_sb<SB>();
   }
 }
 
@@ -776,7 +785,7 @@ __device__ __forceinline__ void k_zsolve128_body(const Geom& g, const SolveArgs&
     z128_exchange<-1, kM>(v, wl, lds + (wv ^ 1) * kRegion128, lane, p, t, real_col);
   }
   __syncthreads();  // the LDS rows are reused by the 64-point transforms
-  if (real_col) z64_fwd_regs<kM>(v, wl, lane, tw);
+  if (real_col) z64_fwd_regs<kM, (kL >= 7)>(v, wl, lane, tw);
   if (a.solve && real_col) {
     const double kxv = signed_k(kx, g.nx) * a.kfac[0];
     const double kyv = signed_k(ky, ny) * a.kfac[1];
@@ -784,7 +793,7 @@ __device__ __forceinline__ void k_zsolve128_body(const Geom& g, const SolveArgs&
     Solver<kL>::solve(v, kxv, kyv, kzv, a.cdelta, a.sigma, a.ds, a.dslam);
   }
   if (real_col) {
-    z64_inv_regs<kM>(v, wl, lane, tw);
+    z64_inv_regs<kM, (kL >= 7)>(v, wl, lane, tw);
     if (p == 1) {
 #pragma unroll
       for (int h = 0; h < kM; ++h) v[h] = twiddle<+1>(v[h], t);
@@ -1070,10 +1079,12 @@ hipError_t zsolve_plan(const Geom& g, const SolveArgs& a, ZPlan* z) {
     z->block = dim3(256);
     return hipSuccess;
   }
-  if (g.nz == 128 && !a.generic && kL >= 5 && kL <= 7 && a.split) {
-    // one workgroup (one column, harmonics split over two waves per z half) per column
-    z->k1 = (const void*)k_zsolve128s<2, S>;
-    z->kb = (const void*)k_zsolve128s_b<2, S>;
+  if (g.nz == 128 && !a.generic && kL >= 5 && a.split) {
+    // one workgroup (one column, harmonics split over two waves per z half) per column; two
+    // waves per SIMD up to Lmax 7, one at Lmax 8 (the 45 harmonics of l = 6..8 need more
+    // than half the register file)
+    z->k1 = (const void*)k_zsolve128s<(kL >= 8 ? 1 : 2), S>;
+    z->kb = (const void*)k_zsolve128s_b<(kL >= 8 ? 1 : 2), S>;
     z->grid = dim3(4 * g.nt * g.ny);
     z->block = dim3(256);
     return hipSuccess;
