@@ -14,8 +14,13 @@ import unet_oracle as U  # noqa: E402
 
 
 @pytest.mark.parametrize("shape,cin,cout", [((2, 4, 6, 8), 16, 16), ((1, 8, 8, 8), 32, 48),
-                                            ((3, 5, 3, 7), 48, 16)])
+                                            ((3, 5, 3, 7), 48, 16), ((1, 8, 16, 16), 64, 32),
+                                            ((2, 4, 4, 16), 32, 128), ((1, 3, 8, 24), 16, 64)])
 def test_conv_and_gradients_match_oracle(wlc, shape, cin, cout):
+    """Shapes with an x edge that is a multiple of 8 (and 32 | voxels) take the transposed
+    weight-gradient kernel, the others the element-wise one; input channels that are a multiple
+    of 32 take the 16x16x32 matrix-core step, 16 and 48 the 16x16x16 one; 16, 32 and 64 or more
+    output channels give 1, 2 and 4 output tiles per workgroup."""
     rng = np.random.default_rng(cin + cout)
     x = rng.standard_normal(shape + (cin,))
     w = rng.standard_normal((cout, 27, cin)) * 0.2

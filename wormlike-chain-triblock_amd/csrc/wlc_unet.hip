@@ -40,6 +40,8 @@ namespace nn {
 typedef unsigned short bf16;
 typedef short bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
 typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // round to nearest even (finite inputs)
 __host__ __device__ inline bf16 f2bf(float f) {
@@ -73,40 +75,165 @@ __device__ __forceinline__ long long neighbour(const Vol& g, long long v, int t)
   return ((b * g.D + zz) * g.H + yy) * g.W + xx;
 }
 
-// Y[v][co] = bias[co] + sum_{t, ci} W[co][t][ci] X[nbr(v, t)][ci]: one wave per 16 voxels x
-// 16 output channels; lane l holds A[l % 16][4 (l / 16) + j] (voxel row, channel group) and
-// B[4 (l / 16) + j][l % 16] (channel group, output column) and receives
-// D[4 (l / 16) + r][l % 16]
-__global__ void __launch_bounds__(64) k_conv_fwd(const bf16* __restrict__ X, const bf16* __restrict__ Wt,
-                                                 const float* __restrict__ bias, float* __restrict__ Y,
-                                                 Vol g, int Cin, int Cout) {
-  const int l = threadIdx.x, row = l & 15, kq = l >> 4;
-  const long long V = g.voxels();
-  const long long v0 = (long long)blockIdx.x * 16;
-  const int co0 = blockIdx.y * 16;
-  const long long v = v0 + row;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  const bf16x4 zero = {0, 0, 0, 0};
+// Y[v][co] = bias[co] + sum_{t, ci} W[co][t][ci] X[nbr(v, t)][ci] (zero outside the grid).
+// Workgroup = 64 consecutive voxels x 16 NT output channels, wave w its voxels 16 w .. 16 w + 15.
+// Matrix-core step (16 voxels x 16 channels x KI input channels): lane l holds the input
+// channels KI/4 (l / 16) .. of voxel l % 16 (A) and of output channel l % 16 (B), contiguous
+// in both tensors, so each operand is one 8- or 16-byte load; it receives D[4 (l / 16) + r]
+// [l % 16].  KI = 32 (v_mfma_f32_16x16x32_bf16) when Cin is a multiple of 32, else 16.  The
+// voxel's coordinates are decoded once; a tap only adds an offset and checks the bounds.
+template <int NT, int KI>
+__global__ void __launch_bounds__(256) k_conv_fwd(const bf16* __restrict__ X, const bf16* __restrict__ Wt,
+                                                  const float* __restrict__ bias, float* __restrict__ Y,
+                                                  Vol g, int Cin, int Cout) {
+  constexpr int CPL = KI / 4;  // input channels per lane and step
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = lane & 15, kq = lane >> 4;
+  const int V = (int)g.voxels();
+  const int vb = blockIdx.x * 64 + 16 * wv;
+  const int v = vb + row;
+  const int co0 = blockIdx.y * 16 * NT;
+  const bool vin = v < V;
+  int x = 0, y = 0, z = 0;
+  if (vin) {
+    x = v % g.W;
+    const int r = v / g.W;
+    y = r % g.H;
+    z = (r / g.H) % g.D;
+  }
+  f32x4 acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const size_t wstride = (size_t)16 * 27 * Cin;  // between the NT output-channel tiles
   for (int t = 0; t < 27; ++t) {
-    const long long nb = v < V ? neighbour(g, v, t) : -1;
-    const bf16* xr = X + (nb < 0 ? 0 : nb) * Cin + 4 * kq;
-    const bf16* wr = Wt + ((long long)(co0 + row) * 27 + t) * Cin + 4 * kq;
-    for (int c0 = 0; c0 < Cin; c0 += 16) {
-      const bf16x4 a = nb >= 0 ? *reinterpret_cast<const bf16x4*>(xr + c0) : zero;
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(wr + c0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const bool ok = vin && (unsigned)(z + dz) < (unsigned)g.D && (unsigned)(y + dy) < (unsigned)g.H &&
+                    (unsigned)(x + dx) < (unsigned)g.W;
+    const int nb = ok ? v + (dz * g.H + dy) * g.W + dx : 0;
+    const bf16* xr = X + (size_t)nb * Cin + CPL * kq;
+    const bf16* wr = Wt + ((size_t)(co0 + row) * 27 + t) * Cin + CPL * kq;
+    for (int c0 = 0; c0 < Cin; c0 += KI) {
+      if constexpr (KI == 32) {
+        const s16x8 a = ok ? *reinterpret_cast<const s16x8*>(xr + c0) : s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const s16x8 b = *reinterpret_cast<const s16x8*>(wr + j * wstride + c0);
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                           __builtin_bit_cast(bf16x8, b), acc[j], 0, 0, 0);
+        }
+      } else {
+        const bf16x4 a = ok ? *reinterpret_cast<const bf16x4*>(xr + c0) : bf16x4{0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const bf16x4 b = *reinterpret_cast<const bf16x4*>(wr + j * wstride + c0);
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+      }
     }
   }
-  const float bv = bias ? bias[co0 + row] : 0.f;
-  for (int r = 0; r < 4; ++r) {
-    const long long vv = v0 + 4 * kq + r;
-    if (vv < V) Y[vv * Cout + co0 + row] = acc[r] + bv;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = co0 + 16 * j + row;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int vv = vb + 4 * kq + r;
+      if (vv < V) Y[(size_t)vv * Cout + co] = acc[j][r] + bv;
+    }
   }
 }
 
-// part[chunk][co][t][ci] = sum_{v in chunk} dY[v][co] X[nbr(v, t)][ci]: one wave per (tap,
-// 16 co, 16 ci) and voxel chunk; lane l holds A[co = l % 16][v = 4 (l / 16) + j] and
-// B[v = 4 (l / 16) + j][ci = l % 16]
+// out[c][v] = in[v][c] (bf16), 32 x 32 tiles through LDS
+__global__ void __launch_bounds__(256) k_transpose(const bf16* __restrict__ in, bf16* __restrict__ out,
+                                                   int V, int C) {
+  __shared__ bf16 tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int v0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int v = v0 + ty + 8 * r, c = c0 + tx;
+    if (v < V && c < C) tile[ty + 8 * r][tx] = in[(size_t)v * C + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int c = c0 + ty + 8 * r, v = v0 + tx;
+    if (v < V && c < C) out[(size_t)c * V + v] = tile[tx][ty + 8 * r];
+  }
+}
+
+// part[chunk][co][t][ci] = sum_{v in chunk} dY[v][co] X[nbr(v, t)][ci] from the transposed
+// operands dYT [Cout][V] and XT [Cin][V]: the voxels are the reduction index, so lane l's
+// eight consecutive voxels 8 (l / 16) .. of a 32-voxel step are one 16-byte load of a
+// channel row.  Needs W % 8 == 0 (eight consecutive voxels lie in one x row: their tap
+// neighbours are the same run shifted by dx, read as the aligned run and its left or right
+// neighbour run) and V, chunk multiples of 32.  One wave per (tap, 16 MC output channels,
+// 16 NC input channels, voxel chunk).
+template <int MC, int NC>
+__global__ void __launch_bounds__(64) k_conv_dw_t(const bf16* __restrict__ dYT, const bf16* __restrict__ XT,
+                                                  float* __restrict__ part, Vol g, int Cin, int Cout, int chunk) {
+  const int lane = threadIdx.x, row = lane & 15, kq = lane >> 4;
+  const int nci = Cin / (16 * NC);
+  int idx = blockIdx.x;
+  const int t = idx % 27;
+  idx /= 27;
+  const int ci0 = (idx % nci) * 16 * NC, co0 = (idx / nci) * 16 * MC;
+  const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+  const int V = (int)g.voxels();
+  const int vb0 = blockIdx.y * chunk;
+  const int ve = vb0 + chunk < V ? vb0 + chunk : V;
+  const int rowshift = (dz * g.H + dy) * g.W;
+  const s16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+  f32x4 acc[MC][NC];
+#pragma unroll
+  for (int i = 0; i < MC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int vb = vb0; vb < ve; vb += 32) {
+    const int v = vb + 8 * kq;  // this lane's eight voxels v .. v + 7: one x-row run
+    const int x = v % g.W;
+    const int r1 = v / g.W;
+    const int y = r1 % g.H, z = (r1 / g.H) % g.D;
+    const bool rowok = (unsigned)(z + dz) < (unsigned)g.D && (unsigned)(y + dy) < (unsigned)g.H;
+    const int base = v + rowshift;  // neighbour run at dx = 0 (a multiple of 8)
+    s16x8 a[MC], b[NC];
+#pragma unroll
+    for (int i = 0; i < MC; ++i) a[i] = *reinterpret_cast<const s16x8*>(dYT + (size_t)(co0 + 16 * i + row) * V + v);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const bf16* p = XT + (size_t)(ci0 + 16 * j + row) * V;
+      const s16x8 c = rowok ? *reinterpret_cast<const s16x8*>(p + base) : zero;
+      if (dx == 0) {
+        b[j] = c;
+      } else if (dx < 0) {  // voxels v - 1 .. v + 6: the last of the run to the left, then c[0..6]
+        const s16x8 l = (rowok && x > 0) ? *reinterpret_cast<const s16x8*>(p + base - 8) : zero;
+        b[j] = s16x8{l[7], c[0], c[1], c[2], c[3], c[4], c[5], c[6]};
+      } else {  // voxels v + 1 .. v + 8: c[1..7], then the first of the run to the right
+        const s16x8 r = (rowok && x + 8 < g.W) ? *reinterpret_cast<const s16x8*>(p + base + 8) : zero;
+        b[j] = s16x8{c[1], c[2], c[3], c[4], c[5], c[6], c[7], r[0]};
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[i]),
+                                                            __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < MC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        part[(((size_t)blockIdx.y * Cout + co0 + 16 * i + 4 * kq + r) * 27 + t) * Cin + ci0 + 16 * j + row] =
+            acc[i][j][r];
+}
+
+// part[chunk][co][t][ci] = sum_{v in chunk} dY[v][co] X[nbr(v, t)][ci] on the channels-last
+// operands, element by element (grids whose x edge is not a multiple of 8, i.e. the coarsest
+// levels of small grids): one wave per (tap, 16 co, 16 ci) and voxel chunk; lane l holds
+// A[co = l % 16][v = 4 (l / 16) + j] and B[v = 4 (l / 16) + j][ci = l % 16]
 __global__ void __launch_bounds__(64) k_conv_dw(const bf16* __restrict__ dY, const bf16* __restrict__ X,
                                                 float* __restrict__ part, Vol g, int Cin, int Cout,
                                                 long long chunk) {
@@ -144,20 +271,50 @@ __global__ void k_sum_chunks(const float* __restrict__ part, float* __restrict__
   out[i] = s;
 }
 
-// db[co] = sum_v dY[v][co] (FP32), one workgroup per channel, fixed order
-__global__ void __launch_bounds__(256) k_bias_grad(const float* __restrict__ dY, float* __restrict__ db,
-                                                   long long V, int C) {
+// bpart[p][c] = sum of dY[v][c] over the voxels of part p (FP32): the 256 threads are
+// 256 / C voxel lanes x C channels (C <= 256), so each pass reads 256 / C whole voxel rows;
+// every lane keeps four partial sums (independent loads in flight), and the sums are added
+// in a fixed order (deterministic)
+__global__ void __launch_bounds__(256) k_bias_partial(const float* __restrict__ dY, float* __restrict__ bpart,
+                                                      int V, int C, int vchunk) {
   __shared__ float red[256];
-  const int co = blockIdx.x;
+  const int nr = 256 / C, c = threadIdx.x % C, r = threadIdx.x / C;
+  const int v0 = blockIdx.x * vchunk, v1 = v0 + vchunk < V ? v0 + vchunk : V;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  if (r < nr) {
+    int v = v0 + r;
+    for (; v + 3 * nr < v1; v += 4 * nr) {
+      s0 += dY[(size_t)v * C + c];
+      s1 += dY[(size_t)(v + nr) * C + c];
+      s2 += dY[(size_t)(v + 2 * nr) * C + c];
+      s3 += dY[(size_t)(v + 3 * nr) * C + c];
+    }
+    for (; v < v1; v += nr) s0 += dY[(size_t)v * C + c];
+  }
+  red[threadIdx.x] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float t = 0.f;
+    for (int k = 0; k < nr; ++k) t += red[k * C + c];
+    bpart[(size_t)blockIdx.x * C + c] = t;
+  }
+}
+
+// db[c] = sum over the np parts of bpart[p][c]: one workgroup per channel, each thread adds
+// the parts t, t + 256, ... in order, then a tree in shared memory (fixed order)
+__global__ void __launch_bounds__(256) k_bias_final(const float* __restrict__ bpart, float* __restrict__ db,
+                                                    int np, int C) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
   float s = 0.f;
-  for (long long v = threadIdx.x; v < V; v += 256) s += dY[v * C + co];
+  for (int p = threadIdx.x; p < np; p += 256) s += bpart[(size_t)p * C + c];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
     if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) db[co] = red[0];
+  if (threadIdx.x == 0) db[c] = red[0];
 }
 
 // W'[ci][26 - t][co] = W[co][t][ci] (bf16)
@@ -394,6 +551,8 @@ struct wlc_unet {
   unsigned char* mask = nullptr;
   bf16* wflip = nullptr;  // flipped weights of one layer
   float* part = nullptr;  // weight-gradient partial sums
+  float* bpart = nullptr; // bias-gradient partial sums
+  bf16 *xt = nullptr, *dyt = nullptr;  // a layer's input and output gradient, transposed
   long long part_n = 0;
   float* lpart = nullptr;
   int adam_t = 0;
@@ -451,6 +610,11 @@ void unet_free(wlc_unet* u) {
   delete u;
 }
 
+// voxel chunks of the weight-gradient reduction (at most 64, about 4096 voxels each)
+int wgrad_chunks(long long V) { return (int)std::min<long long>(64, std::max<long long>(1, V / 4096)); }
+// partial sums of the bias gradient (at most; about 256 voxels each)
+constexpr int kBiasParts = 1024;
+
 // activation, gradient and scratch buffers for grid edge n and batch B
 int alloc_acts(wlc_unet* u, int n, int B) {
   free_acts(u);
@@ -458,14 +622,16 @@ int alloc_acts(wlc_unet* u, int n, int B) {
   u->B = B;
   int rc;
 #define UA(ptr, cnt) if ((rc = ualloc(u->acts, &(ptr), (cnt)))) { free_acts(u); return rc; }
-  long long maxpart = 0;
+  long long maxpart = 0, maxx = 0, maxdy = 0;
   for (const Layer& l : u->layers) {
     const long long nw = (long long)l.cout * 27 * l.cin;
     const long long V = u->vol(l.level).voxels();
-    const long long nch = std::min<long long>(64, std::max<long long>(1, V / 4096));
-    maxpart = std::max(maxpart, nch * nw);
+    maxpart = std::max(maxpart, wgrad_chunks(V) * nw);
+    maxx = std::max(maxx, V * l.cin);
+    maxdy = std::max(maxdy, V * l.cout);
   }
-  UA(u->part, maxpart); UA(u->lpart, 256);
+  UA(u->part, maxpart); UA(u->lpart, 256); UA(u->bpart, (long long)kBiasParts * 256);
+  UA(u->xt, maxx); UA(u->dyt, maxdy);
   const long long V0 = u->vol(0).voxels(), V1 = u->vol(1).voxels(), V2 = u->vol(2).voxels(), V3 = u->vol(3).voxels();
   UA(u->x0, V0 * 16); UA(u->a0a, V0 * 32); UA(u->s0, V0 * 32); UA(u->p1, V1 * 32); UA(u->a1a, V1 * 64);
   UA(u->s1, V1 * 64); UA(u->p2, V2 * 64); UA(u->a2a, V2 * 128); UA(u->s2, V2 * 128); UA(u->p3, V3 * 128);
@@ -485,31 +651,81 @@ int alloc_acts(wlc_unet* u, int n, int B) {
   return WLC_OK;
 }
 
-// y = conv(X) (+ bias) of layer li on level lv; X has l.cin channels
-int conv_forward(wlc_unet* u, const bf16* X, const bf16* W, const float* bias, float* Y, Vol g, int cin,
-                 int cout) {
+// launches shared by the network and the convolution test entry point (stream st)
+
+// y = conv(X) (+ bias); X has cin channels, Y cout (multiples of 16)
+int launch_conv_fwd(hipStream_t st, const bf16* X, const bf16* W, const float* bias, float* Y, Vol g, int cin,
+                    int cout) {
   const long long V = g.voxels();
-  hipLaunchKernelGGL(k_conv_fwd, dim3(blocks(V, 16), cout / 16), dim3(64), 0, u->st, X, W, bias, Y, g, cin, cout);
+  const int nt = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+  const dim3 grid(blocks(V, 64), cout / (16 * nt)), block(256);
+#define CONV_FWD(NT, KI) hipLaunchKernelGGL((k_conv_fwd<NT, KI>), grid, block, 0, st, X, W, bias, Y, g, cin, cout)
+  if (cin % 32 == 0) {
+    if (nt == 4) CONV_FWD(4, 32); else if (nt == 2) CONV_FWD(2, 32); else CONV_FWD(1, 32);
+  } else {
+    if (nt == 4) CONV_FWD(4, 16); else if (nt == 2) CONV_FWD(2, 16); else CONV_FWD(1, 16);
+  }
+#undef CONV_FWD
   UCHK(hipGetLastError());
   return WLC_OK;
+}
+
+// dW [cout][27][cin] and db [cout] of one convolution (overwritten) from its input X
+// (channels-last bf16), the output gradient dYb (bf16) and dYf (FP32).  Scratch: xt >= V cin
+// and dyt >= V cout bf16, part >= wgrad_chunks(V) cout 27 cin floats, bpart >= kBiasParts cout.
+int launch_conv_wgrad(hipStream_t st, const bf16* X, const bf16* dYb, const float* dYf, Vol g, int cin, int cout,
+                      bf16* xt, bf16* dyt, float* part, float* bpart, float* dW, float* db) {
+  const long long V = g.voxels();
+  const long long nw = (long long)cout * 27 * cin;
+  const int nch = wgrad_chunks(V);
+  int nrun;
+  if (g.W % 8 == 0 && V % 32 == 0) {
+    const long long chunk = ((V + nch - 1) / nch + 31) / 32 * 32;
+    nrun = (int)((V + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_transpose, dim3(blocks(V, 32), blocks(cin, 32)), dim3(256), 0, st, X, xt, (int)V, cin);
+    hipLaunchKernelGGL(k_transpose, dim3(blocks(V, 32), blocks(cout, 32)), dim3(256), 0, st, dYb, dyt, (int)V, cout);
+    const int mc = cout % 32 == 0 ? 2 : 1, nc = cin % 32 == 0 ? 2 : 1;
+    const dim3 grid(27 * (cout / (16 * mc)) * (cin / (16 * nc)), nrun), block(64);
+#define CONV_DW(MC, NC) \
+  hipLaunchKernelGGL((k_conv_dw_t<MC, NC>), grid, block, 0, st, dyt, xt, part, g, cin, cout, (int)chunk)
+    if (mc == 2) {
+      if (nc == 2) CONV_DW(2, 2); else CONV_DW(2, 1);
+    } else {
+      if (nc == 2) CONV_DW(1, 2); else CONV_DW(1, 1);
+    }
+#undef CONV_DW
+  } else {
+    const long long chunk = ((V + nch - 1) / nch + 15) / 16 * 16;
+    nrun = (int)((V + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_conv_dw, dim3(27 * (cout / 16) * (cin / 16), nrun), dim3(64), 0, st, dYb, X, part, g, cin,
+                       cout, chunk);
+  }
+  UCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_chunks, dim3(blocks(nw)), dim3(256), 0, st, part, dW, nw, nrun);
+  const int nbp = (int)std::min<long long>(kBiasParts, std::max<long long>(1, V / 256));
+  const int vchunk = (int)((V + nbp - 1) / nbp);
+  hipLaunchKernelGGL(k_bias_partial, dim3(nbp), dim3(256), 0, st, dYf, bpart, (int)V, cout, vchunk);
+  hipLaunchKernelGGL(k_bias_final, dim3(cout), dim3(256), 0, st, bpart, db, nbp, cout);
+  UCHK(hipGetLastError());
+  return WLC_OK;
+}
+
+int conv_forward(wlc_unet* u, const bf16* X, const bf16* W, const float* bias, float* Y, Vol g, int cin,
+                 int cout) {
+  return launch_conv_fwd(u->st, X, W, bias, Y, g, cin, cout);
 }
 
 // gradients of layer li: dW, db into u->g; dX (FP32, overwritten) when dX != nullptr
 int conv_backward(wlc_unet* u, const Layer& l, const bf16* X, float* dX) {
   const int li = (int)(&l - u->layers.data());
   const Vol g = u->vol(l.level);
-  const long long V = g.voxels();
-  const int nch = (int)std::min<long long>(64, std::max<long long>(1, V / 4096));
-  const long long chunk = ((V + nch - 1) / nch + 15) / 16 * 16;
   const long long nw = (long long)l.cout * 27 * l.cin;
-  hipLaunchKernelGGL(k_conv_dw, dim3(27 * (l.cout / 16) * (l.cin / 16), nch), dim3(64), 0, u->st, u->dyb[li], X,
-                     u->part, g, l.cin, l.cout, chunk);
-  UCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_chunks, dim3(blocks(nw)), dim3(256), 0, u->st, u->part, u->g + l.woff, nw, nch);
-  hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, u->st, u->dy[li], u->g + l.boff, V, l.cout);
+  int rc = launch_conv_wgrad(u->st, X, u->dyb[li], u->dy[li], g, l.cin, l.cout, u->xt, u->dyt, u->part, u->bpart,
+                             u->g + l.woff, u->g + l.boff);
+  if (rc) return rc;
   if (dX) {
     hipLaunchKernelGGL(k_flip_weights, dim3(blocks(nw)), dim3(256), 0, u->st, u->pb + l.woff, u->wflip, l.cin, l.cout);
-    int rc = conv_forward(u, u->dyb[li], u->wflip, nullptr, dX, g, l.cout, l.cin);
+    rc = conv_forward(u, u->dyb[li], u->wflip, nullptr, dX, g, l.cout, l.cin);
     if (rc) return rc;
   }
   UCHK(hipGetLastError());
@@ -635,6 +851,8 @@ int wlc_unet_create(int n, int batch, unsigned long long seed, wlc_unet** out) {
   if (!out) return ufail(WLC_EINVAL, "out is NULL");
   *out = nullptr;
   if (n < 8 || n % 8 || batch < 1) return ufail(WLC_EINVAL, "grid edge must be a multiple of 8 and batch >= 1");
+  if ((long long)batch * n * n * n * 256 >= (1LL << 31))
+    return ufail(WLC_EINVAL, "batch x grid too large (the kernels index an activation tensor with 32 bits)");
   wlc_unet* u = new wlc_unet();
   u->n = n;
   u->B = batch;
@@ -693,6 +911,8 @@ void wlc_unet_destroy(wlc_unet* u) { unet_free(u); }
 int wlc_unet_resize(wlc_unet* u, int n, int batch) {
   if (!u) return ufail(WLC_EINVAL, "NULL network");
   if (n < 8 || n % 8 || batch < 1) return ufail(WLC_EINVAL, "grid edge must be a multiple of 8 and batch >= 1");
+  if ((long long)batch * n * n * n * 256 >= (1LL << 31))
+    return ufail(WLC_EINVAL, "batch x grid too large (the kernels index an activation tensor with 32 bits)");
   UCHK(hipSetDevice(u->device));
   UCHK(hipStreamSynchronize(u->st));
   return alloc_acts(u, n, batch);
@@ -781,48 +1001,52 @@ int wlc_unet_set_params(wlc_unet* u, const float* in) {
 // network): y = conv(x), dx, dw, db.  cin, cout multiples of 16.
 int wlc_test_conv3d(int B, int D, int H, int W, int cin, int cout, const float* x, const float* w,
                     const float* b, const float* dy, float* y, float* dx, float* dw, float* db) {
-  if (cin % 16 || cout % 16 || B < 1 || D < 1 || H < 1 || W < 1 || !x || !w || !b || !dy || !y || !dx || !dw || !db)
+  if (cin % 16 || cout % 16 || cin > 256 || cout > 256 || B < 1 || D < 1 || H < 1 || W < 1 || !x || !w || !b ||
+      !dy || !y || !dx || !dw || !db)
     return ufail(WLC_EINVAL, "bad arguments");
   const Vol g{B, D, H, W};
   const long long V = g.voxels(), nw = (long long)cout * 27 * cin;
-  const int nch = (int)std::min<long long>(64, std::max<long long>(1, V / 4096));
-  const long long chunk = ((V + nch - 1) / nch + 15) / 16 * 16;
+  if (V * std::max(cin, cout) >= (1LL << 31)) return ufail(WLC_EINVAL, "tensor too large");
+  const int nch = wgrad_chunks(V);
   std::vector<bf16> hx((size_t)(V * cin)), hw((size_t)nw), hdy((size_t)(V * cout));
   for (size_t i = 0; i < hx.size(); ++i) hx[i] = f2bf(x[i]);
   for (size_t i = 0; i < hw.size(); ++i) hw[i] = f2bf(w[i]);
   for (size_t i = 0; i < hdy.size(); ++i) hdy[i] = f2bf(dy[i]);
-  bf16 *dX = nullptr, *dW = nullptr, *dDY = nullptr, *dWf = nullptr;
-  float *dB = nullptr, *dYo = nullptr, *dDX = nullptr, *dDW = nullptr, *dDB = nullptr, *dPart = nullptr, *dDYf = nullptr;
+  bf16 *dX = nullptr, *dW = nullptr, *dDY = nullptr, *dWf = nullptr, *dXT = nullptr, *dDYT = nullptr;
+  float *dB = nullptr, *dYo = nullptr, *dDX = nullptr, *dDW = nullptr, *dDB = nullptr, *dPart = nullptr,
+        *dDYf = nullptr, *dBpart = nullptr;
   hipError_t e = hipSuccess;
   auto M = [&](void** q, size_t bytes) { if (e == hipSuccess) e = hipMalloc(q, bytes); };
   M((void**)&dX, sizeof(bf16) * hx.size()); M((void**)&dW, sizeof(bf16) * nw); M((void**)&dDY, sizeof(bf16) * hdy.size());
   M((void**)&dWf, sizeof(bf16) * nw); M((void**)&dB, sizeof(float) * cout); M((void**)&dYo, sizeof(float) * V * cout);
   M((void**)&dDX, sizeof(float) * V * cin); M((void**)&dDW, sizeof(float) * nw); M((void**)&dDB, sizeof(float) * cout);
   M((void**)&dPart, sizeof(float) * nw * nch); M((void**)&dDYf, sizeof(float) * V * cout);
+  M((void**)&dXT, sizeof(bf16) * hx.size()); M((void**)&dDYT, sizeof(bf16) * hdy.size());
+  M((void**)&dBpart, sizeof(float) * kBiasParts * cout);
   if (e == hipSuccess) e = hipMemcpy(dX, hx.data(), sizeof(bf16) * hx.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dW, hw.data(), sizeof(bf16) * nw, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dDY, hdy.data(), sizeof(bf16) * hdy.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dB, b, sizeof(float) * cout, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dDYf, dy, sizeof(float) * V * cout, hipMemcpyHostToDevice);
+  int rc = WLC_OK;
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_conv_fwd, dim3(blocks(V, 16), cout / 16), dim3(64), 0, nullptr, dX, dW, dB, dYo, g, cin, cout);
-    hipLaunchKernelGGL(k_conv_dw, dim3(27 * (cout / 16) * (cin / 16), nch), dim3(64), 0, nullptr, dDY, dX, dPart, g,
-                       cin, cout, chunk);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks(nw)), dim3(256), 0, nullptr, dPart, dDW, nw, nch);
-    hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, nullptr, dDYf, dDB, V, cout);
-    hipLaunchKernelGGL(k_flip_weights, dim3(blocks(nw)), dim3(256), 0, nullptr, dW, dWf, cin, cout);
-    hipLaunchKernelGGL(k_conv_fwd, dim3(blocks(V, 16), cin / 16), dim3(64), 0, nullptr, dDY, dWf, nullptr, dDX, g,
-                       cout, cin);
-    e = hipGetLastError();
+    rc = launch_conv_fwd(nullptr, dX, dW, dB, dYo, g, cin, cout);
+    if (!rc) rc = launch_conv_wgrad(nullptr, dX, dDY, dDYf, g, cin, cout, dXT, dDYT, dPart, dBpart, dDW, dDB);
+    if (!rc) {
+      hipLaunchKernelGGL(k_flip_weights, dim3(blocks(nw)), dim3(256), 0, nullptr, dW, dWf, cin, cout);
+      rc = launch_conv_fwd(nullptr, dDY, dWf, nullptr, dDX, g, cout, cin);
+    }
+    if (!rc) e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e == hipSuccess) e = hipMemcpy(y, dYo, sizeof(float) * V * cout, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(dx, dDX, sizeof(float) * V * cin, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(dw, dDW, sizeof(float) * nw, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(db, dDB, sizeof(float) * cout, hipMemcpyDeviceToHost);
+  if (!rc && e == hipSuccess) e = hipDeviceSynchronize();
+  if (!rc && e == hipSuccess) e = hipMemcpy(y, dYo, sizeof(float) * V * cout, hipMemcpyDeviceToHost);
+  if (!rc && e == hipSuccess) e = hipMemcpy(dx, dDX, sizeof(float) * V * cin, hipMemcpyDeviceToHost);
+  if (!rc && e == hipSuccess) e = hipMemcpy(dw, dDW, sizeof(float) * nw, hipMemcpyDeviceToHost);
+  if (!rc && e == hipSuccess) e = hipMemcpy(db, dDB, sizeof(float) * cout, hipMemcpyDeviceToHost);
   for (void* q : {(void*)dX, (void*)dW, (void*)dDY, (void*)dWf, (void*)dB, (void*)dYo, (void*)dDX, (void*)dDW,
-                  (void*)dDB, (void*)dPart, (void*)dDYf})
+                  (void*)dDB, (void*)dPart, (void*)dDYf, (void*)dXT, (void*)dDYT, (void*)dBpart})
     if (q) (void)hipFree(q);
+  if (rc) return rc;
   if (e != hipSuccess) return ufail(WLC_EHIP, std::string("test conv3d: ") + hipGetErrorString(e));
   return WLC_OK;
 }
