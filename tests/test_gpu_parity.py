@@ -198,7 +198,7 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     column split over two waves (per z half at nz = 128); the unsplit kernel computes the same step
     (equal up to rounding: the split kernel passes k = 0 through identity-angle rotations
     instead of a diagonal shortcut).  nx/2 + 1 = 5 leaves padding columns in the last tile.
-    At nz = 64 the whole-tile split kernel (the default) and the two-column split kernel
+    At nz = 64 the whole-tile and half-tile split kernels and the two-column split kernel
     run the same operations (equal up to the compiler's multiply-add contraction).  At
     other nz (24: the two parts share a wave; 32) the generic kernel splits Lmax 4..7 over
     two threads per k-point."""
@@ -206,7 +206,7 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     rng = np.random.default_rng(31)
     w = 1.5 * rng.standard_normal((3,) + shape)
     out = []
-    for split in ("0", "1", "2") if shape[0] == 64 else ("0", "1"):
+    for split in ("0", "1", "2", "3") if shape[0] == 64 else ("0", "1"):
         monkeypatch.setenv("WLC_ZSOLVE_SPLIT", split)
         g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, keep_qstar=True)
         g.set_w(w)
@@ -218,11 +218,12 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     (q0, s0, p0, Q0), (q1, s1, p1, Q1) = out[:2]
     assert _rel(q1, q0) < 1e-13 and _rel(s1, s0) < 1e-13 and _rel(p1, p0) < 1e-13
     assert abs(Q1 - Q0) / abs(Q0) < 1e-13
-    if len(out) == 3:
-        q2, s2, p2, Q2 = out[2]
-        print("tile vs two-column kernel:", _rel(q2, q1), _rel(s2, s1), _rel(p2, p1), flush=True)
-        assert _rel(q2, q1) < 1e-13 and _rel(s2, s1) < 1e-13 and _rel(p2, p1) < 1e-13
-        assert abs(Q2 - Q1) / abs(Q1) < 1e-13
+    for k, name in ((2, "tile"), (3, "half-tile")):
+        if len(out) > k:
+            q2, s2, p2, Q2 = out[k]
+            print(name, "vs two-column kernel:", _rel(q2, q1), _rel(s2, s1), _rel(p2, p1), flush=True)
+            assert _rel(q2, q1) < 1e-13 and _rel(s2, s1) < 1e-13 and _rel(p2, p1) < 1e-13
+            assert abs(Q2 - Q1) / abs(Q1) < 1e-13
 
 
 @pytest.mark.parametrize("keep", [False, True])

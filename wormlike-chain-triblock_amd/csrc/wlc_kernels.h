@@ -66,9 +66,11 @@ struct SolveArgs {
   int generic;         // 1: force the generic (any nz) kernel
   int occ;             // nz = 64 kernel: target waves per SIMD (1 or 2)
   int split;           // harmonics of a k-point split over two waves / threads by degree
-                       // (nz = 64 and 128: Lmax 5-7; other nz: Lmax 4-7): 0 off; nz = 64:
-                       // 1 two columns per workgroup, 2 a whole 4-kx tile per workgroup
-                       // (FP64 spectrum; FP32 runs as 1); elsewhere 1 or 2 select the split
+                       // (nz = 64: Lmax 5-7; nz = 128: Lmax 5-8; other nz: Lmax 4-7): 0 off;
+                       // nz = 64: 1 two columns per workgroup, loaded directly, 2 a whole 4-kx
+                       // tile per workgroup, 3 half a tile per workgroup, both through LDS
+                       // (2 and 3: FP64 spectrum; FP32 runs as 1); elsewhere any nonzero
+                       // value selects the split
 };
 
 // Kernel C: inverse y and x (C2R) passes, scaling by 1/N, store of q(s).

@@ -50,7 +50,7 @@ ZOpts zsolve_opts(int nz, int L, int spec32) {
   }
   {
     const int v = env_int("WLC_ZSOLVE_SPLIT", o.split);
-    o.split = (v == 1 || v == 2) ? v : 0;
+    o.split = (v >= 1 && v <= 3) ? v : 0;
   }
   {
     const int v = env_int("WLC_ZSOLVE_TX", o.tx);

@@ -440,16 +440,20 @@ __global__ void __launch_bounds__(256, OCC) k_zsolve64s_b(const ZJob* __restrict
 constexpr int kNBt = (kNV + kHB - 1) / kHB;
 constexpr int kTileLds = 2 * kKxTile * kRegionS;  // dynamic LDS, double2 units
 
-template <class S>
+// NCOL columns per part (4: the whole tile, k_zsolve64t; 2: half a tile, k_zsolve64p):
+// thread gi of the part moves element (z = gi / NCOL, column gi % NCOL) of each harmonic, at
+// offset 4 z + column of the part's first column (hp)
+template <int NCOL = kKxTile, class S>
 __device__ __forceinline__ void zt_forward(double2 (&v)[kNV], double2* glds, double2* wl, int lane,
                                            int gi, int nh, bool run, const double2 (&tw)[8],
                                            const S* __restrict__ hp, long long istride, bool io) {
   const int h = lane >> 3, j = lane & 7;
-  const int tz = gi >> 2, tc = gi & 3;
+  const int tz = gi / NCOL, tc = gi % NCOL;
+  const int go = kKxTile * tz + tc;  // the element's offset in a harmonic's tile
   double2 nxt[kHB];
 #pragma unroll
   for (int q = 0; q < kHB; ++q)
-    nxt[q] = (io && q < nh) ? spec_ld(hp + (long long)q * istride + gi) : make_double2(0.0, 0.0);
+    nxt[q] = (io && q < nh) ? spec_ld(hp + (long long)q * istride + go) : make_double2(0.0, 0.0);
 #pragma unroll
   for (int bb = 0; bb < kNBt; ++bb) {
     const int b0 = bb * kHB;
@@ -461,7 +465,7 @@ __device__ __forceinline__ void zt_forward(double2 (&v)[kNV], double2* glds, dou
 #pragma unroll
       for (int q = 0; q < kHB; ++q) {
         const int hq = b0 + kHB + q;
-        nxt[q] = (io && hq < nh) ? spec_ld(hp + (long long)hq * istride + gi) : make_double2(0.0, 0.0);
+        nxt[q] = (io && hq < nh) ? spec_ld(hp + (long long)hq * istride + go) : make_double2(0.0, 0.0);
       }
     }
     __syncthreads();
@@ -490,13 +494,14 @@ __device__ __forceinline__ void zt_forward(double2 (&v)[kNV], double2* glds, dou
   }
 }
 
-template <class S>
+template <int NCOL = kKxTile, class S>
 __device__ __forceinline__ void zt_inverse(const double2 (&v)[kNV], double2* glds, double2* wl,
                                            int lane, int gi, int nh, bool run,
                                            const double2 (&tw)[8], S* __restrict__ hp,
                                            long long istride, bool io) {
   const int h = lane >> 3, j = lane & 7;
-  const int tz = gi >> 2, tc = gi & 3;
+  const int tz = gi / NCOL, tc = gi % NCOL;
+  const int go = kKxTile * tz + tc;
 #pragma unroll
   for (int bb = 0; bb < kNBt; ++bb) {
     const int b0 = bb * kHB;
@@ -527,7 +532,7 @@ __device__ __forceinline__ void zt_inverse(const double2 (&v)[kNV], double2* gld
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (io && b0 + q < nh)
-        spec_st(hp + (long long)(b0 + q) * istride + gi, glds[tc * kRegionS + q * kPadBatch + pad8(tz)]);
+        spec_st(hp + (long long)(b0 + q) * istride + go, glds[tc * kRegionS + q * kPadBatch + pad8(tz)]);
   }
 }
 
@@ -608,6 +613,88 @@ __global__ void __launch_bounds__(512) k_zsolve64t_b(const ZJob* __restrict__ jo
   const ZJob& j = jobs[blockIdx.z];
   if (j.skip && *j.skip) return;
   k_zsolve64t_body<S>(j.g, j.a);
+}
+
+// Half-tile variant (k_zsolve64p): the same split, coalesced through LDS, with a workgroup
+// of four waves per two adjacent columns of a tile (wave w = 2 p + c: part p of column
+// 2 half + c), so that two workgroups fit on a CU (217 VGPRs, two waves per SIMD) and one
+// workgroup's loads and stores overlap the other's transforms and solve; the whole-tile
+// kernel has a single eight-wave workgroup per CU, whose phases cannot overlap.  Each
+// part moves 32 contiguous bytes per z of every harmonic; the two halves of a tile are
+// blocks b and b + 8, consecutive on one XCD (the same L2).
+template <class S>
+__device__ __forceinline__ void k_zsolve64p_body(const Geom& g, const SolveArgs& a) {
+  extern __shared__ double2 lds[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int part = wv >> 1, c = wv & 1, gi = threadIdx.x & 127;
+  const int ny = g.ny, nxh = g.nxh, nt = g.nt;
+  const int nb = gridDim.x, b = blockIdx.x;
+  int tl, half;
+  if ((nb & 15) == 0) {
+    const int t = b >> 3;
+    half = t & 1;
+    tl = (t >> 1) * 8 + (b & 7);
+  } else {
+    half = b & 1;
+    tl = b >> 1;
+  }
+  const int t4 = tl % nt, ky = tl / nt;
+  if (t4 * kKxTile + 2 * half >= nxh) return;  // both columns are padding (uniform)
+  const int kx = t4 * kKxTile + 2 * half + c;
+  const bool real_col = kx < nxh;                                       // wave-uniform
+  const bool io = t4 * kKxTile + 2 * half + (gi & 1) < nxh;             // the column this thread moves
+  const bool active = a.solve && real_col;
+  const long long istride = spec_index(1, 0, 0, 0, ny, nt, 64);
+  const int nh = part ? PartHi::NH : PartLo::NH;
+  const int hb0 = part ? PartHi::B0 : PartLo::B0;
+  S* hp = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, 64) + 2 * half +
+          (long long)hb0 * istride;
+  double2* glds = lds + part * (2 * kRegionS);
+  double2* wl = glds + c * kRegionS;
+  double2* xch = lds + c * kRegionS;  // the Lo wave's region of column c
+  double2 tw[8];
+  {
+    const int jt = lane & 7;
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) tw[k2] = g.az.tw[jt * k2];
+  }
+  const KFrame f(signed_k(kx, g.nx) * a.kfac[0], signed_k(ky, ny) * a.kfac[1],
+                 signed_k(lane, 64) * a.kfac[2], a.sigma, a.ds);
+  double2 v[kNV];
+  zt_forward<2>(v, glds, wl, lane, gi, nh, real_col, tw, hp, istride, io);
+  if (active) {
+    if (part == 0) {
+      PartLo::to_frame(v, f);
+      tri_lower_forward<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+    } else {
+      PartHi::to_frame(v, f);
+    }
+  }
+  __syncthreads();
+  if (part == 1 && active) tri_upper<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+  __syncthreads();
+  if (active) {
+    if (part == 0) {
+      tri_lower_back<kL, kLS>(v, a.cdelta, a.dslam, f.gk, xch, lane);
+      PartLo::from_frame(v, f);
+    } else {
+      PartHi::from_frame(v, f);
+    }
+  }
+  zt_inverse<2>(v, glds, wl, lane, gi, nh, real_col, tw, hp, istride, io);
+}
+
+template <class S>
+__global__ void __launch_bounds__(256, 2) k_zsolve64p(Geom g, SolveArgs a) {
+  k_zsolve64p_body<S>(g, a);
+}
+
+// the same for one instance of a batch per blockIdx.z
+template <class S>
+__global__ void __launch_bounds__(256, 2) k_zsolve64p_b(const ZJob* __restrict__ jobs) {
+  const ZJob& j = jobs[blockIdx.z];
+  if (j.skip && *j.skip) return;
+  k_zsolve64p_body<S>(j.g, j.a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1062,6 +1149,15 @@ hipError_t zsolve_plan(const Geom& g, const SolveArgs& a, ZPlan* z) {
   if constexpr (std::is_same<S, double2>::value) {
     // one workgroup of eight waves per 4-kx tile; FP64 spectrum only: with the FP32 half
     // spectrum (2 KB tiles) the two-column kernel below is faster (64^3/L6: 69 vs 93 us)
+    if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split == 3) {
+      // two workgroups of four waves per 4-kx tile (half a tile each)
+      z->k1 = (const void*)k_zsolve64p<S>;
+      z->kb = (const void*)k_zsolve64p_b<S>;
+      z->grid = dim3(2 * g.nt * g.ny);
+      z->block = dim3(256);
+      z->lds = sizeof(double2) * (size_t)(2 * 2 * kRegionS);
+      return hipSuccess;
+    }
     if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split == 2) {
       z->k1 = (const void*)k_zsolve64t<S>;
       z->kb = (const void*)k_zsolve64t_b<S>;
