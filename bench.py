@@ -270,11 +270,11 @@ def c4_report(kind, rows, extra):
         "config": {"workload": "linear ABC wormlike triblocks from seed 3 (the 48-instance batching "
                                "draw): cubic grid edge {24,32,48,64}, Lmax {4,6}, f_A, f_B ~ "
                                "U[0.2,0.45], chiN_ij ~ U[12,30], lambda ~ U[0.25,4], box ~ U[3,6], "
-                               "Ns 100, w = 0.1 N(0,1); 20 simple-mixing iterations (0.05) then "
-                               "Anderson (10 histories, inputs mixed, 0.1), at most max_iter "
-                               "iterations (520: 500 Anderson)", "tol": extra.pop("tol"),
+                               "Ns 100, w = 0.1 N(0,1); n_simple simple-mixing iterations (step "
+                               "lam_s) then Anderson (n_hist histories, form 1: inputs mixed, step "
+                               "lam_a), at most max_iter iterations", "tol": extra.pop("tol"),
                    "max_iter": extra.pop("max_iter"),
-                   "mixing": "simple then Anderson"},
+                   "mixing": extra.pop("mixing", dict(MIX))},
         "per_instance": [{k: (round(v, 10) if isinstance(v, float) else v) for k, v in r.items()}
                          for r in rows],
     }
@@ -370,9 +370,16 @@ def bench_batch_c4(args, W):
     iteration in which it converged (or stopped)."""
     idx_cases = c4_cases(args)
     cases = [c for _, c in idx_cases]
-    b = W.Batch(cases, W.scft_opts(max_iter=args.max_iter, n_simple=MIX["n_simple"], n_hist=MIX["n_hist"],
-                                   lam_simple=MIX["lam_s"], lam_anderson=MIX["lam_a"],
-                                   form=MIX["form"], tol=args.tol))
+    mix = dict(MIX)
+    if args.warm:
+        mix["n_simple"] = 0  # the proposal is close to the solution: Anderson from the start
+    if args.n_simple is not None:
+        mix["n_simple"] = args.n_simple
+    if args.lam_anderson is not None:
+        mix["lam_a"] = args.lam_anderson
+    b = W.Batch(cases, W.scft_opts(max_iter=args.max_iter, n_simple=mix["n_simple"], n_hist=mix["n_hist"],
+                                   lam_simple=mix["lam_s"], lam_anderson=mix["lam_a"],
+                                   form=mix["form"], tol=args.tol))
     t_net = 0.0
     init = [c["w"] for c in cases]
     if args.warm:
@@ -413,7 +420,7 @@ def bench_batch_c4(args, W):
               {"batch_wall_s": round(wall + t_net, 3),
                "amortised_s_per_instance": round((wall + t_net) / len(rows), 4),
                "network_s": round(t_net, 3), "launches_per_iteration": launches, "tol": args.tol,
-               "max_iter": args.max_iter})
+               "max_iter": args.max_iter, "mixing": mix})
 
 
 def main():
@@ -449,6 +456,10 @@ def main():
     ap.add_argument("--max-iter", type=int, default=520, help="iteration cap of the time-to-residual runs")
     ap.add_argument("--c4-edges", default="", help="only instances with these grid edges (e.g. 24,32)")
     ap.add_argument("--warm", default="", help="weights (.npy) of the warm-start network for --batch-c4")
+    ap.add_argument("--n-simple", type=int, default=None,
+                    help="--batch-c4: simple-mixing iterations before Anderson (default 20, 0 with --warm)")
+    ap.add_argument("--lam-anderson", type=float, default=None,
+                    help="--batch-c4: Anderson step (default 0.1, inputs mixed)")
     ap.add_argument("--warm-train", action="store_true",
                     help="generate training data with the batched solver and train the warm-start "
                          "network; weights to --out")

@@ -191,7 +191,8 @@ def test_fused_inverse_rhs_is_bit_identical(wlc, case, monkeypatch):
 
 
 @pytest.mark.parametrize("shape,L", [((64, 6, 8), (5, 7, 6)), ((128, 4, 8), (5, 7, 6)),
-                                     ((128, 4, 6), (8, 6, 8)),
+                                     ((128, 4, 6), (8, 6, 8)), ((48, 6, 8), (6, 4, 7)),
+                                     ((48, 4, 6), (5, 7, 4)),
                                      ((24, 6, 12), (4, 5, 7)), ((32, 4, 6), (6, 4, 4))])
 def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     """Lmax 5..7 at nz = 64 and 5..8 at nz = 128 runs the z solve with the harmonics of a
@@ -199,14 +200,18 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     (equal up to rounding: the split kernel passes k = 0 through identity-angle rotations
     instead of a diagonal shortcut).  nx/2 + 1 = 5 leaves padding columns in the last tile.
     At nz = 64 the whole-tile and half-tile split kernels and the two-column split kernel
-    run the same operations (equal up to the compiler's multiply-add contraction).  At
+    run the same operations (equal up to the compiler's multiply-add contraction); at nz = 48
+    the half-tile kernel (wave-local 6 x 8 transform) is compared with the generic split
+    kernel (block-wide Stockham transform).  At
     other nz (24: the two parts share a wave; 32) the generic kernel splits Lmax 4..7 over
     two threads per k-point."""
     box, Nblk, lam = (3.0, 2.0, 5.0), (3, 4, 3), (1.2, 0.6, 1.0)
     rng = np.random.default_rng(31)
     w = 1.5 * rng.standard_normal((3,) + shape)
     out = []
-    for split in ("0", "1", "2", "3") if shape[0] == 64 else ("0", "1"):
+    monkeypatch.setenv("WLC_GENERIC_ZSOLVE", "0")
+    splits = {64: ("0", "1", "2", "3"), 48: ("0", "1", "3")}.get(shape[0], ("0", "1"))
+    for split in splits:
         monkeypatch.setenv("WLC_ZSOLVE_SPLIT", split)
         g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, keep_qstar=True)
         g.set_w(w)
@@ -218,12 +223,12 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     (q0, s0, p0, Q0), (q1, s1, p1, Q1) = out[:2]
     assert _rel(q1, q0) < 1e-13 and _rel(s1, s0) < 1e-13 and _rel(p1, p0) < 1e-13
     assert abs(Q1 - Q0) / abs(Q0) < 1e-13
-    for k, name in ((2, "tile"), (3, "half-tile")):
-        if len(out) > k:
-            q2, s2, p2, Q2 = out[k]
-            print(name, "vs two-column kernel:", _rel(q2, q1), _rel(s2, s1), _rel(p2, p1), flush=True)
-            assert _rel(q2, q1) < 1e-13 and _rel(s2, s1) < 1e-13 and _rel(p2, p1) < 1e-13
-            assert abs(Q2 - Q1) / abs(Q1) < 1e-13
+    names = {"2": "tile", "3": "half-tile"}
+    for k in range(2, len(out)):
+        q2, s2, p2, Q2 = out[k]
+        print(names[splits[k]], "vs split kernel 1:", _rel(q2, q1), _rel(s2, s1), _rel(p2, p1), flush=True)
+        assert _rel(q2, q1) < 1e-13 and _rel(s2, s1) < 1e-13 and _rel(p2, p1) < 1e-13
+        assert abs(Q2 - Q1) / abs(Q1) < 1e-13
 
 
 @pytest.mark.parametrize("keep", [False, True])
