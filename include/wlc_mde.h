@@ -196,7 +196,10 @@ typedef struct wlc_scft_opts {
                            times the best since the last restart the history restarts from the
                            current entry and the step halves (no counterpart in the reference,
                            whose step is fixed) */
-  int reserved[3];      /* must be zero */
+  int no_compact;       /* 0 (default): once at most half of the instances in the launch
+                           sequence still advance, rebuild it over those (fewer launches per
+                           iteration, same results); 1: keep every instance in it */
+  int reserved[2];      /* must be zero */
   double lam_max;       /* adaptive mode: largest Anderson step (0: 1.0) */
 } wlc_scft_opts;
 

@@ -194,3 +194,34 @@ def test_converged_instances_stop_advancing(wlc):
     assert np.array_equal(b.get_w(1), w1)
     assert b.status(0)["iterations"] == 8
     b.close()
+
+
+def test_compacted_launch_sequence_matches_the_full_one(wlc):
+    """Once at most half of the instances still advance, the batch rebuilds its launch
+    sequence (and graph) over those alone; the advancing instances run the same kernels on
+    the same data, so every field, iteration count and free energy equals a batch that keeps
+    every instance in the sequence, bit for bit."""
+    cases = []
+    for c in (CASES[0], CASES[5], CASES[1], CASES[5], CASES[5]):
+        chi = np.zeros((3, 3))
+        nb = len(c["Nblk"])
+        chi[:nb, :nb] = _chi(nb)
+        cases.append(dict(c, chi=chi))
+    out = []
+    for compact in (True, False):
+        b = wlc.Batch(cases, wlc.scft_opts(max_iter=12, n_simple=2, n_hist=3, tol=1e-8, compact=compact))
+        for i, c in enumerate(cases):
+            if i in (1, 3, 4):
+                # homopolymers at w = 0 converge at the first iteration: 2 of 5 remain, so the
+                # sequence is rebuilt over those two
+                b.set_w(i, np.zeros((1,) + c["shape"]))
+            else:
+                b.set_w(i, 0.5 * _w(c, 5 + i))
+        b.iterate(4, use_graph=True)
+        b.iterate(10, use_graph=True)
+        out.append([(b.get_w(i), b.status(i)) for i in range(len(cases))])
+        b.close()
+    for (w0, s0), (w1, s1) in zip(*out):
+        assert np.array_equal(w0, w1)
+        assert s0["iterations"] == s1["iterations"] and s0["converged"] == s1["converged"]
+        assert s0["F"] == s1["F"] and s0["lnQ"] == s1["lnQ"]

@@ -60,6 +60,9 @@ CASES = [
     ((128, 4, 6), (6.0, 2.0, 2.5), (3, 4), (8, 7), (0.5, 1.0), 1),
     # ... nx/2 + 1 = 5 columns (a half-padded and an all-padding workgroup), Lmax 5 and 3
     ((128, 4, 8), (5.0, 2.0, 3.0), (4, 3), (5, 3), (1.0, 0.4), 0),
+    # 24 x 24 and 48 x 48 planes (compile-time Stockham passes: radix 8, 3 and 4, 4, 3)
+    ((4, 24, 24), (2.0, 4.0, 4.0), (3, 4, 3), (4, 3, 5), (1.0, 0.6, 1.4), 0),
+    ((2, 48, 48), (1.5, 5.0, 5.0), (4, 3), (3, 4), (0.8, 1.2), 0),
     # 128 x 128 planes (128 KB of LDS per workgroup, radix 16 x 8 passes)
     ((4, 128, 128), (2.0, 6.0, 6.0), (3, 3), (2, 1), (0.5, 1.5), 0),
     ((2, 128, 128), (1.0, 4.0, 5.0), (2, 3, 2), (3, 2, 3), (1.0, 0.5, 2.0), 0),

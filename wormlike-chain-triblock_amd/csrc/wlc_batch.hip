@@ -55,6 +55,7 @@ struct MixOpts {
   double lam_s, lam_a, tol;
   int adaptive;
   double lam_max;
+  int compact;  // rebuild the launch sequence over the advancing instances (host side)
 };
 
 struct DeltaJob {
@@ -566,6 +567,11 @@ struct wlc_batch {
   char* djobs = nullptr;
   size_t jobs_bytes = 0;
   std::vector<wlc::Op> sweep_ops, mix_ops;
+  // instances in the launch sequence: all at first; when at most half of them are still
+  // advancing, the sequence is rebuilt over those alone (a finished instance's launches
+  // would only return at once, but each still costs its launch)
+  std::vector<char> use;
+  int included = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool swept = false;
@@ -715,13 +721,16 @@ DensJob dens_job(const Inst& I, int s, const int* skip) {
 // t <= Ns, then the fused inverse + next right-hand side (or the inverse alone at its
 // last step); backward sweeps add the density term of the slice just written
 void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir) {
-  const int n = (int)b->in.size();
+  std::vector<int> idx;
+  for (int i = 0; i < (int)b->in.size(); ++i)
+    if (b->use[i]) idx.push_back(i);
+  const unsigned n = (unsigned)idx.size();
   int Tmax = 0;
-  for (const Inst& I : b->in) Tmax = std::max(Tmax, I.Ns);
+  for (int i : idx) Tmax = std::max(Tmax, b->in[i].Ns);
   {  // q(0) / q*(Ns) = delta_{i0}
     std::vector<DeltaJob> dj;
     int gy = 0, gx = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i : idx) {
       const Inst& I = b->in[i];
       const int s0 = dir > 0 ? 0 : I.Ns;
       DeltaJob d{};
@@ -738,7 +747,7 @@ void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir)
   if (dir < 0) {
     std::vector<FillJob> fj;
     long long mx = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i : idx) {
       const Inst& I = b->in[i];
       fj.push_back(FillJob{I.rho, (long long)I.p.nblk * I.hs, b->dskip + i});
       mx = std::max(mx, (long long)I.p.nblk * I.hs);
@@ -746,7 +755,7 @@ void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir)
     emit_group(t, ops, kOpFill, 0, fj, dim3((unsigned)std::min<long long>((mx + 255) / 256, 65536), 1, n));
     std::vector<DensJob> dj;
     int gx = 0, gz = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i : idx) {
       const Inst& I = b->in[i];
       dj.push_back(dens_job(I, I.Ns, b->dskip + i));
       gx = std::max(gx, (I.g.nx * I.g.ny / 2 + 255) / 256);
@@ -758,7 +767,7 @@ void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir)
   {
     Grouper<PlaneJob> gr;
     std::vector<int> orders;
-    for (int i = 0; i < n; ++i) {
+    for (int i : idx) {
       const Inst& I = b->in[i];
       const int s = dir > 0 ? 1 : I.Ns - 1;
       PlaneJob pj{};
@@ -774,7 +783,7 @@ void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir)
   }
   for (int step = 1; step <= Tmax; ++step) {
     Grouper<ZJob> gz;
-    for (int i = 0; i < n; ++i) {
+    for (int i : idx) {
       const Inst& I = b->in[i];
       if (step > I.Ns) continue;
       const int s = dir > 0 ? step : I.Ns - step;
@@ -788,7 +797,7 @@ void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir)
     Grouper<PlaneJob> gc, gca;
     std::vector<DensJob> dj;
     int dgx = 0, dgz = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i : idx) {
       const Inst& I = b->in[i];
       if (step > I.Ns) continue;
       const int s = dir > 0 ? step : I.Ns - step;
@@ -822,11 +831,14 @@ void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir)
 
 // Q, totden and phi of every instance (the handle's density_common)
 void build_observables(const wlc_batch* b, JobTable& t, std::vector<Op>& ops) {
-  const int n = (int)b->in.size();
+  std::vector<int> idx;
+  for (int i = 0; i < (int)b->in.size(); ++i)
+    if (b->use[i]) idx.push_back(i);
+  const unsigned n = (unsigned)idx.size();
   std::vector<SumJob> sq, sr;
   std::vector<PhiJob> pj;
   int gx = 0, gy = 0;
-  for (int i = 0; i < n; ++i) {
+  for (int i : idx) {
     const Inst& I = b->in[i];
     MixState* st = b->dstate + i;
     SumJob a{};
@@ -864,9 +876,12 @@ void build_observables(const wlc_batch* b, JobTable& t, std::vector<Op>& ops) {
 }
 
 void build_mix(const wlc_batch* b, JobTable& t, std::vector<Op>& ops) {
-  const int n = (int)b->in.size();
+  std::vector<int> idx;
+  for (int i = 0; i < (int)b->in.size(); ++i)
+    if (b->use[i]) idx.push_back(i);
+  const unsigned n = (unsigned)idx.size();
   std::vector<MixJob> mj;
-  for (int i = 0; i < n; ++i) {
+  for (int i : idx) {
     const Inst& I = b->in[i];
     MixJob m{};
     m.g = I.g;
@@ -951,6 +966,35 @@ int run_ops(wlc_batch* b, const std::vector<Op>& ops) {
   return WLC_OK;
 }
 
+// the launch sequences over the included instances (b->use): sweeps + observables
+// (wlc_batch_sweep), and the same followed by the field update (one SCFT iteration); the job
+// table is uploaded again and the captured graph dropped (recaptured at the next iteration).
+// Called with no work in flight on the batch's stream.
+int rebuild_ops(wlc_batch* b) {
+  JobTable t;
+  std::vector<Op> sw, mx;
+  build_sweep(b, t, sw, +1);
+  build_sweep(b, t, sw, -1);
+  build_observables(b, t, sw);
+  mx = sw;
+  build_mix(b, t, mx);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  if (b->djobs) (void)hipFree(b->djobs);
+  b->djobs = nullptr;
+  b->jobs_bytes = t.bytes.size();
+  if (hipMalloc((void**)&b->djobs, std::max<size_t>(b->jobs_bytes, 16)) != hipSuccess ||
+      hipMemcpy(b->djobs, t.bytes.data(), b->jobs_bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return host_fail(WLC_ENOMEM, "job table upload failed");
+  b->sweep_ops.swap(sw);
+  b->mix_ops.swap(mx);
+  b->included = 0;
+  for (char u : b->use) b->included += u ? 1 : 0;
+  return WLC_OK;
+}
+
 void free_batch(wlc_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
@@ -1028,6 +1072,7 @@ int wlc_batch_create(const wlc_params* params, const double* chi, int n, const w
   if (o.n_simple < 0 || o.max_iter < 1 || (o.form != 0 && o.form != 1))
     return host_fail(WLC_EINVAL, "bad mixing options");
   if (o.adaptive != 0 && o.adaptive != 1) return host_fail(WLC_EINVAL, "adaptive must be 0 or 1");
+  if (o.no_compact != 0 && o.no_compact != 1) return host_fail(WLC_EINVAL, "no_compact must be 0 or 1");
   for (int r : o.reserved)
     if (r) return host_fail(WLC_EINVAL, "reserved option fields must be zero");
   for (int i = 0; i < n; ++i) {
@@ -1047,7 +1092,7 @@ int wlc_batch_create(const wlc_params* params, const double* chi, int n, const w
     return rc;
   }
   b->mo = MixOpts{o.n_simple, o.n_hist,   o.form, o.max_iter, o.lam_simple, o.lam_anderson,
-                  o.tol,      o.adaptive, o.lam_max > 0.0 ? o.lam_max : 1.0};
+                  o.tol,      o.adaptive, o.lam_max > 0.0 ? o.lam_max : 1.0, o.no_compact ? 0 : 1};
   if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
     free_batch(b);
     return host_fail(WLC_EHIP, "stream creation failed");
@@ -1170,19 +1215,10 @@ int wlc_batch_create(const wlc_params* params, const double* chi, int n, const w
     free_batch(b);
     return host_fail(WLC_EHIP, "state initialisation failed");
   }
-  // the launch sequences: sweeps + observables (wlc_batch_sweep), and the same followed by the
-  // field update (one SCFT iteration)
-  JobTable t;
-  build_sweep(b, t, b->sweep_ops, +1);
-  build_sweep(b, t, b->sweep_ops, -1);
-  build_observables(b, t, b->sweep_ops);
-  b->mix_ops = b->sweep_ops;
-  build_mix(b, t, b->mix_ops);
-  b->jobs_bytes = t.bytes.size();
-  if (hipMalloc((void**)&b->djobs, b->jobs_bytes) != hipSuccess ||
-      hipMemcpy(b->djobs, t.bytes.data(), b->jobs_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+  b->use.assign(n, 1);
+  if (int rc = rebuild_ops(b)) {
     free_batch(b);
-    return host_fail(WLC_ENOMEM, "job table upload failed");
+    return rc;
   }
   b->done_ms.assign(n, -1.0);
   *out = b;
@@ -1296,6 +1332,24 @@ int wlc_batch_iterate(wlc_batch* b, int niter, int use_graph, int* n_active) {
       if (!b->hskip[i]) ++active;
     }
     if (active == 0) break;
+    if (b->mo.compact && 2 * active <= b->included) {
+      // at most half of the launch sequence's instances still advance: rebuild it over those
+      for (int i = 0; i < n; ++i) b->use[i] = b->hskip[i] ? 0 : 1;
+      if (int rc = rebuild_ops(b)) return rc;
+      if (use_graph) {
+        BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        int rc2 = run_ops(b, b->mix_ops);
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->st, &gr);
+        if (rc2) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return rc2;
+        }
+        BCHK(ec);
+        b->graph = gr;
+        BCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      }
+    }
   }
   b->swept = true;
   if (n_active) *n_active = active;

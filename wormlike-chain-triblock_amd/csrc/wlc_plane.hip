@@ -377,9 +377,12 @@ static void* inv_rhs_kernel(int order) {
   return nullptr;
 }
 
-// 32 x 32 planes take compile-time Stockham passes (64 x 64 planes have their own
-// kernels), everything else the runtime kernels
-static bool specialised(const Geom& g) { return g.nx == 32 && g.ny == 32; }
+// 24 x 24, 32 x 32 and 48 x 48 planes (the batching workload's other grids) take
+// compile-time Stockham passes (64 x 64 and 128 x 128 planes have their own kernels),
+// everything else the runtime kernels
+static bool specialised(const Geom& g) {
+  return g.nx == g.ny && (g.nx == 24 || g.nx == 32 || g.nx == 48);
+}
 
 static size_t plane_lds_bytes(const Geom& g) {
   const size_t n = (size_t)g.ny * g.nxh;
@@ -443,7 +446,11 @@ static void* pick_plane_kernel_t(const Geom& g, int kind, int order, size_t* lds
   (kind == 0 ? rhs_kernel<NX, NY, MAXE, S, B>(order)                                      \
    : kind == 1 ? (B ? (void*)k_ifft_xy_b<NX, NY, MAXE, S> : (void*)k_ifft_xy<NX, NY, MAXE, S>) \
                : inv_rhs_kernel<NX, NY, MAXE, S, B>(order))
-  if (specialised(g)) return WLC_PICK(32, 32, 0);
+  if (specialised(g)) {
+    if (g.nx == 24) return WLC_PICK(24, 24, 0);
+    if (g.nx == 48) return WLC_PICK(48, 48, 0);
+    return WLC_PICK(32, 32, 0);
+  }
   switch (pick_maxe((long long)g.ny * g.nxh, kThreads)) {
     case 8: return WLC_PICK(0, 0, 8);
     case 16: return WLC_PICK(0, 0, 16);

@@ -49,7 +49,8 @@ class ScftOpts(ctypes.Structure):
         ("max_iter", ctypes.c_int), ("n_simple", ctypes.c_int), ("n_hist", ctypes.c_int),
         ("form", ctypes.c_int), ("lam_simple", ctypes.c_double),
         ("lam_anderson", ctypes.c_double), ("tol", ctypes.c_double),
-        ("adaptive", ctypes.c_int), ("reserved", ctypes.c_int * 3), ("lam_max", ctypes.c_double),
+        ("adaptive", ctypes.c_int), ("no_compact", ctypes.c_int), ("reserved", ctypes.c_int * 2),
+        ("lam_max", ctypes.c_double),
     ]
 
 
@@ -334,11 +335,12 @@ def step_noalloc(params, direction, s, hist_ptrs, w_ptr, out_ptr, stream=None):
 
 
 def scft_opts(max_iter=500, n_simple=20, n_hist=10, form=0, lam_simple=0.05, lam_anderson=0.1,
-              tol=1e-6, adaptive=False, lam_max=1.0):
+              tol=1e-6, adaptive=False, lam_max=1.0, compact=True):
     o = ScftOpts()
     o.max_iter, o.n_simple, o.n_hist, o.form = max_iter, n_simple, n_hist, form
     o.lam_simple, o.lam_anderson, o.tol = lam_simple, lam_anderson, tol
     o.adaptive, o.lam_max = 1 if adaptive else 0, lam_max
+    o.no_compact = 0 if compact else 1
     return o
 
 
