@@ -35,6 +35,24 @@ def test_conv_and_gradients_match_oracle(wlc, shape, cin, cout):
         assert np.abs(got - ref).max() <= 2e-5 * np.abs(ref).max()
 
 
+def test_lds_staged_convolution_equals_the_direct_one(wlc, monkeypatch):
+    """The forward kernel with the weights staged through LDS (default for Cin a multiple of
+    32) runs the same matrix-core steps in the same order as the direct kernel
+    (WLC_CONV_LDS=0): outputs and input gradients are equal bit for bit."""
+    rng = np.random.default_rng(4)
+    shape, cin, cout = (1, 8, 16, 16), 64, 128
+    x = rng.standard_normal(shape + (cin,))
+    w = rng.standard_normal((cout, 27, cin)) * 0.2
+    b = rng.standard_normal(cout)
+    dy = rng.standard_normal(shape + (cout,))
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("WLC_CONV_LDS", flag)
+        out.append(wlc.test_conv3d(x, w, b, dy))
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
+
+
 def _inputs(n, B, seed):
     rng = np.random.default_rng(seed)
     x = np.concatenate([rng.standard_normal((B, n, n, n, 3)),

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <random>
@@ -129,6 +130,84 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const bf16* __restrict__ X, co
           acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
         }
       }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = co0 + 16 * j + row;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int vv = vb + 4 * kq + r;
+      if (vv < V) Y[(size_t)vv * Cout + co] = acc[j][r] + bv;
+    }
+  }
+}
+
+// The same convolution (Cin a multiple of 32) with the weights staged through LDS and the
+// next step's operands in flight: per K step (tap t, 32 input channels) the workgroup's
+// 16 NT x 32 weight block (NT KB) is loaded once by 64 NT threads into one of two LDS buffers
+// instead of once per wave, every wave reads its B fragments from there (16 lanes of equal
+// kq read 256 contiguous bytes), and the global loads of step k + 1 (weights and the gathered
+// input run) are issued before the matrix-core steps of step k.  One barrier per step: a
+// thread reaching barrier k has finished its reads of step k - 1, so the buffer written next
+// is free.
+template <int NT>
+__global__ void __launch_bounds__(256) k_conv_fwd_l(const bf16* __restrict__ X, const bf16* __restrict__ Wt,
+                                                    const float* __restrict__ bias, float* __restrict__ Y,
+                                                    Vol g, int Cin, int Cout) {
+  __shared__ s16x8 wbuf[2][4 * 16 * NT];  // [kq][output channel] 16-byte pieces
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = lane & 15, kq = lane >> 4;
+  const int V = (int)g.voxels();
+  const int vb = blockIdx.x * 64 + 16 * wv;
+  const int v = vb + row;
+  const int co0 = blockIdx.y * 16 * NT;
+  const bool vin = v < V;
+  int x = 0, y = 0, z = 0;
+  if (vin) {
+    x = v % g.W;
+    const int r = v / g.W;
+    y = r % g.H;
+    z = (r / g.H) % g.D;
+  }
+  const int nch = Cin >> 5;
+  const int K = 27 * nch;
+  // staging role: thread st < 64 NT moves the 8 channels 8 (st & 3) .. of output channel st >> 2
+  const int st = threadIdx.x;
+  const bool stager = st < 64 * NT;
+  const int sco = st >> 2, skq = st & 3;
+  const bf16* wrow = Wt + (size_t)(co0 + (stager ? sco : 0)) * 27 * Cin + 8 * skq;
+  const s16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+  auto load_w = [&](int ks) -> s16x8 {
+    const int t = ks / nch, c0 = (ks - t * nch) << 5;
+    return stager ? *reinterpret_cast<const s16x8*>(wrow + (size_t)t * Cin + c0) : zero;
+  };
+  auto load_a = [&](int ks) -> s16x8 {
+    const int t = ks / nch, c0 = (ks - t * nch) << 5;
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const bool ok = vin && (unsigned)(z + dz) < (unsigned)g.D && (unsigned)(y + dy) < (unsigned)g.H &&
+                    (unsigned)(x + dx) < (unsigned)g.W;
+    const int nb = ok ? v + (dz * g.H + dy) * g.W + dx : 0;
+    return ok ? *reinterpret_cast<const s16x8*>(X + (size_t)nb * Cin + c0 + 8 * kq) : zero;
+  };
+  f32x4 acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  s16x8 wn = load_w(0), an = load_a(0);
+  for (int ks = 0; ks < K; ++ks) {
+    const s16x8 wc = wn, ac = an;
+    if (stager) wbuf[ks & 1][skq * (16 * NT) + sco] = wc;
+    if (ks + 1 < K) {
+      wn = load_w(ks + 1);
+      an = load_a(ks + 1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const s16x8 b = wbuf[ks & 1][kq * (16 * NT) + 16 * j + row];
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ac),
+                                                       __builtin_bit_cast(bf16x8, b), acc[j], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -651,6 +730,13 @@ int alloc_acts(wlc_unet* u, int n, int B) {
   return WLC_OK;
 }
 
+// the LDS-staged forward convolution (default; WLC_CONV_LDS=0 selects the direct one, for
+// measurements and tests)
+bool conv_lds_on() {
+  const char* e = std::getenv("WLC_CONV_LDS");
+  return !(e && e[0] == '0');
+}
+
 // launches shared by the network and the convolution test entry point (stream st)
 
 // y = conv(X) (+ bias); X has cin channels, Y cout (multiples of 16)
@@ -660,12 +746,18 @@ int launch_conv_fwd(hipStream_t st, const bf16* X, const bf16* W, const float* b
   const int nt = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
   const dim3 grid(blocks(V, 64), cout / (16 * nt)), block(256);
 #define CONV_FWD(NT, KI) hipLaunchKernelGGL((k_conv_fwd<NT, KI>), grid, block, 0, st, X, W, bias, Y, g, cin, cout)
+#define CONV_FWD_L(NT) hipLaunchKernelGGL((k_conv_fwd_l<NT>), grid, block, 0, st, X, W, bias, Y, g, cin, cout)
   if (cin % 32 == 0) {
-    if (nt == 4) CONV_FWD(4, 32); else if (nt == 2) CONV_FWD(2, 32); else CONV_FWD(1, 32);
+    if (conv_lds_on()) {
+      if (nt == 4) CONV_FWD_L(4); else if (nt == 2) CONV_FWD_L(2); else CONV_FWD_L(1);
+    } else {
+      if (nt == 4) CONV_FWD(4, 32); else if (nt == 2) CONV_FWD(2, 32); else CONV_FWD(1, 32);
+    }
   } else {
     if (nt == 4) CONV_FWD(4, 16); else if (nt == 2) CONV_FWD(2, 16); else CONV_FWD(1, 16);
   }
 #undef CONV_FWD
+#undef CONV_FWD_L
   UCHK(hipGetLastError());
   return WLC_OK;
 }
