@@ -203,9 +203,9 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     (equal up to rounding: the split kernel passes k = 0 through identity-angle rotations
     instead of a diagonal shortcut).  nx/2 + 1 = 5 leaves padding columns in the last tile.
     At nz = 64 the whole-tile and half-tile split kernels and the two-column split kernel
-    run the same operations (equal up to the compiler's multiply-add contraction); at nz = 48
-    the half-tile kernel (wave-local 6 x 8 transform) is compared with the generic split
-    kernel (block-wide Stockham transform).  At
+    run the same operations (equal up to the compiler's multiply-add contraction); at nz = 24,
+    32 and 48 the half-tile kernel (wave-local 3, 4 or 6 x 8 transform) is compared with the
+    generic split kernel (block-wide Stockham transform).  At
     other nz (24: the two parts share a wave; 32) the generic kernel splits Lmax 4..7 over
     two threads per k-point."""
     box, Nblk, lam = (3.0, 2.0, 5.0), (3, 4, 3), (1.2, 0.6, 1.0)
@@ -213,7 +213,7 @@ def test_split_z_solve_matches_one_wave_kernel(wlc, monkeypatch, shape, L):
     w = 1.5 * rng.standard_normal((3,) + shape)
     out = []
     monkeypatch.setenv("WLC_GENERIC_ZSOLVE", "0")
-    splits = {64: ("0", "1", "2", "3"), 48: ("0", "1", "3")}.get(shape[0], ("0", "1"))
+    splits = {64: ("0", "1", "2", "3")}.get(shape[0], ("0", "1", "3") if shape[0] in (24, 32, 48) else ("0", "1"))
     for split in splits:
         monkeypatch.setenv("WLC_ZSOLVE_SPLIT", split)
         g = wlc.WormlikeMDE(shape, box, Nblk, L, lam, keep_qstar=True)

@@ -698,19 +698,22 @@ __global__ void __launch_bounds__(256, 2) k_zsolve64p_b(const ZJob* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// nz = 48, 4 <= Lmax <= 7: the half-tile split kernel with a wave-local 48-point transform
+// nz = 24, 32, 48, 4 <= Lmax <= 7: the half-tile split kernel with a wave-local transform
 // ---------------------------------------------------------------------------
 // Same workgroup shape as k_zsolve64p (four waves: two columns x two degree parts, two
 // workgroups per CU; the halves of a tile are blocks b and b + 8 on one XCD), lane = z for
-// lanes 0..47 (lanes 48..63 hold zeros through the solve).  The 48-point transform of a
-// batch of eight harmonics is a four-step 6 x 8 in the wave's LDS region: with
-// n = j + 6 m and k = k1 + 8 k2, a DFT-8 over m for each (harmonic, j) (48 lanes), the
-// twiddle W48^(j k1), then a DFT-6 over j for each (harmonic, k1) (64 lanes).  The generic
-// kernel it replaces reads 16 of every 64 bytes of a tile per thread and runs runtime-planned
-// block-wide Stockham passes.
-constexpr int kNz48 = 48;
-constexpr int kPad48 = 54;  // padded units per harmonic: pad8(47) + 1 = 53, rounded up
-constexpr int kRegion48 = (kHB * kPad48 + 2) > (kXch + 2) ? (kHB * kPad48 + 2) : (kXch + 2);
+// lanes 0..NZ-1 (the others hold zeros through the solve).  The NZ-point transform of a
+// batch of eight harmonics is a four-step N1 x 8 (N1 = NZ / 8 = 3, 4, 6) in the wave's LDS
+// region: with n = j + N1 m and k = k1 + 8 k2, a DFT-8 over m for each (harmonic, j) (NZ
+// lanes), the twiddle W_NZ^(j k1), then a DFT-N1 over j for each (harmonic, k1) (64 lanes).
+// The generic kernel it replaces reads 16 of every 64 bytes of a tile per thread and runs
+// runtime-planned block-wide Stockham passes.
+template <int NZ>
+struct HalfTile {
+  static constexpr int N1 = NZ / 8;
+  static constexpr int kPad = NZ + NZ / 8;  // padded units per harmonic (pad8(NZ - 1) < kPad)
+  static constexpr int kRegion = (kHB * kPad + 2) > (kXch + 2) ? (kHB * kPad + 2) : (kXch + 2);
+};
 
 // X[k] = sum_n x[n] exp(DIR 2 pi i n k / 6): two DFT-3 and one radix-2 step
 template <int DIR>
@@ -731,38 +734,46 @@ __device__ __forceinline__ void dft6(double2 (&v)[6]) {
   v[5] = csub(e[2], o2);
 }
 
-// the 48-point transforms of the eight harmonics in region wl (element z of slot h at
-// wl[h kPad48 + pad8(z)]), in place; DIR < 0 forward, DIR > 0 inverse (unnormalised)
-template <int DIR>
-__device__ __forceinline__ void z48_batch(double2* wl, int lane, const double2* __restrict__ twz) {
-  if (lane < kNz48) {
-    const int h = lane / 6, j = lane - 6 * (lane / 6);
+template <int N, int DIR>
+__device__ __forceinline__ void small_dft(double2 (&v)[N]) {
+  if constexpr (N == 6) dft6<DIR>(v);
+  else Dft<N, DIR>::run(v);
+}
+
+// the NZ-point transforms of the eight harmonics in region wl (element z of slot h at
+// wl[h kPad + pad8(z)]), in place; DIR < 0 forward, DIR > 0 inverse (unnormalised)
+template <int NZ, int DIR>
+__device__ __forceinline__ void zh_batch(double2* wl, int lane, const double2* __restrict__ twz) {
+  constexpr int N1 = HalfTile<NZ>::N1, P = HalfTile<NZ>::kPad;
+  if (lane < NZ) {
+    const int h = lane / N1, j = lane - N1 * (lane / N1);
     double2 u[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) u[m] = wl[h * kPad48 + pad8(j + 6 * m)];
+    for (int m = 0; m < 8; ++m) u[m] = wl[h * P + pad8(j + N1 * m)];
     Dft<8, DIR>::run(u);
 #pragma unroll
     for (int k1 = 1; k1 < 8; ++k1) u[k1] = twiddle<DIR>(u[k1], twz[j * k1]);
-    // the positions j + 6 k1 this lane writes are the ones it read
+    // the positions j + N1 k1 this lane writes are the ones it read
 #pragma unroll
-    for (int k1 = 0; k1 < 8; ++k1) wl[h * kPad48 + pad8(j + 6 * k1)] = u[k1];
+    for (int k1 = 0; k1 < 8; ++k1) wl[h * P + pad8(j + N1 * k1)] = u[k1];
   }
   wave_sync();
   {
     const int h = lane >> 3, k1 = lane & 7;
-    double2 u[6];
+    double2 u[N1];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) u[j] = wl[h * kPad48 + pad8(j + 6 * k1)];
-    dft6<DIR>(u);
+    for (int j = 0; j < N1; ++j) u[j] = wl[h * P + pad8(j + N1 * k1)];
+    small_dft<N1, DIR>(u);
     wave_sync();  // every lane has read its inputs before any output overwrites them
 #pragma unroll
-    for (int k2 = 0; k2 < 6; ++k2) wl[h * kPad48 + pad8(k1 + 8 * k2)] = u[k2];
+    for (int k2 = 0; k2 < N1; ++k2) wl[h * P + pad8(k1 + 8 * k2)] = u[k2];
   }
   wave_sync();
 }
 
-template <class S>
-__device__ __forceinline__ void k_zsolve48p_body(const Geom& g, const SolveArgs& a) {
+template <int NZ, class S>
+__device__ __forceinline__ void k_zsolvehp_body(const Geom& g, const SolveArgs& a) {
+  constexpr int kPadZ = HalfTile<NZ>::kPad, kRegionZ = HalfTile<NZ>::kRegion;
   extern __shared__ double2 lds[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int part = wv >> 1, c = wv & 1, gi = threadIdx.x & 127;
@@ -783,21 +794,21 @@ __device__ __forceinline__ void k_zsolve48p_body(const Geom& g, const SolveArgs&
   const bool real_col = kx < nxh;  // wave-uniform
   // thread gi of a part moves element (z = gi / 2, column gi % 2) of each harmonic
   const int tz = gi >> 1, tc = gi & 1;
-  const bool io = tz < kNz48 && t4 * kKxTile + 2 * half + tc < nxh;
+  const bool io = tz < NZ && t4 * kKxTile + 2 * half + tc < nxh;
   const int go = kKxTile * tz + tc;
   const bool active = a.solve && real_col;
-  const long long istride = spec_index(1, 0, 0, 0, ny, nt, kNz48);
+  const long long istride = spec_index(1, 0, 0, 0, ny, nt, NZ);
   const int nh = part ? PartHi::NH : PartLo::NH;
   const int hb0 = part ? PartHi::B0 : PartLo::B0;
-  S* hp = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, kNz48) + 2 * half +
+  S* hp = reinterpret_cast<S*>(a.spec) + spec_index(0, ky, t4 * kKxTile, 0, ny, nt, NZ) + 2 * half +
           (long long)hb0 * istride;
-  double2* glds = lds + part * (2 * kRegion48);
-  double2* wl = glds + c * kRegion48;
-  double2* xch = lds + c * kRegion48;  // the Lo wave's region of column c
-  const double2* twz = g.az.tw;        // exp(-2 pi i t / 48)
-  const int zl = lane < kNz48 ? lane : 0;
+  double2* glds = lds + part * (2 * kRegionZ);
+  double2* wl = glds + c * kRegionZ;
+  double2* xch = lds + c * kRegionZ;  // the Lo wave's region of column c
+  const double2* twz = g.az.tw;        // exp(-2 pi i t / NZ)
+  const int zl = lane < NZ ? lane : 0;
   const KFrame f(signed_k(kx, g.nx) * a.kfac[0], signed_k(ky, ny) * a.kfac[1],
-                 signed_k(zl, kNz48) * a.kfac[2], a.sigma, a.ds);
+                 signed_k(zl, NZ) * a.kfac[2], a.sigma, a.ds);
   double2 v[kNV];
 #pragma unroll
   for (int q = 0; q < kNV; ++q) v[q] = make_double2(0.0, 0.0);
@@ -811,10 +822,10 @@ __device__ __forceinline__ void k_zsolve48p_body(const Geom& g, const SolveArgs&
   for (int bb = 0; bb < kNBt; ++bb) {
     const int b0 = bb * kHB;
     __syncthreads();  // every wave of the part has read the previous batch out of its region
-    if (tz < kNz48) {
+    if (tz < NZ) {
 #pragma unroll
       for (int q = 0; q < kHB; ++q)
-        if (b0 + q < nh) glds[tc * kRegion48 + q * kPad48 + pad8(tz)] = nxt[q];
+        if (b0 + q < nh) glds[tc * kRegionZ + q * kPadZ + pad8(tz)] = nxt[q];
     }
     if (bb + 1 < kNBt) {
 #pragma unroll
@@ -825,11 +836,11 @@ __device__ __forceinline__ void k_zsolve48p_body(const Geom& g, const SolveArgs&
     }
     __syncthreads();
     if (real_col && b0 < nh) {
-      z48_batch<-1>(wl, lane, twz);
-      if (lane < kNz48) {
+      zh_batch<NZ, -1>(wl, lane, twz);
+      if (lane < NZ) {
 #pragma unroll
         for (int q = 0; q < kHB; ++q)
-          if (b0 + q < kNV && b0 + q < nh) v[b0 + q] = wl[q * kPad48 + pad8(lane)];
+          if (b0 + q < kNV && b0 + q < nh) v[b0 + q] = wl[q * kPadZ + pad8(lane)];
       }
     }
   }
@@ -858,33 +869,33 @@ __device__ __forceinline__ void k_zsolve48p_body(const Geom& g, const SolveArgs&
     const int b0 = bb * kHB;
     __syncthreads();  // the previous batch's tiles (and the crossing chains) have been read
     if (real_col && b0 < nh) {
-      if (lane < kNz48) {
+      if (lane < NZ) {
 #pragma unroll
         for (int q = 0; q < kHB; ++q)
-          if (b0 + q < kNV && b0 + q < nh) wl[q * kPad48 + pad8(lane)] = v[b0 + q];
+          if (b0 + q < kNV && b0 + q < nh) wl[q * kPadZ + pad8(lane)] = v[b0 + q];
       }
       wave_sync();
-      z48_batch<+1>(wl, lane, twz);
+      zh_batch<NZ, +1>(wl, lane, twz);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kHB; ++q)
       if (io && b0 + q < nh)
-        spec_st(hp + (long long)(b0 + q) * istride + go, glds[tc * kRegion48 + q * kPad48 + pad8(tz)]);
+        spec_st(hp + (long long)(b0 + q) * istride + go, glds[tc * kRegionZ + q * kPadZ + pad8(tz)]);
   }
 }
 
-template <class S>
-__global__ void __launch_bounds__(256, 2) k_zsolve48p(Geom g, SolveArgs a) {
-  k_zsolve48p_body<S>(g, a);
+template <int NZ, class S>
+__global__ void __launch_bounds__(256, 2) k_zsolvehp(Geom g, SolveArgs a) {
+  k_zsolvehp_body<NZ, S>(g, a);
 }
 
 // the same for one instance of a batch per blockIdx.z
-template <class S>
-__global__ void __launch_bounds__(256, 2) k_zsolve48p_b(const ZJob* __restrict__ jobs) {
+template <int NZ, class S>
+__global__ void __launch_bounds__(256, 2) k_zsolvehp_b(const ZJob* __restrict__ jobs) {
   const ZJob& j = jobs[blockIdx.z];
   if (j.skip && *j.skip) return;
-  k_zsolve48p_body<S>(j.g, j.a);
+  k_zsolvehp_body<NZ, S>(j.g, j.a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1339,13 +1350,23 @@ hipError_t zsolve_plan(const Geom& g, const SolveArgs& a, ZPlan* z) {
   if constexpr (std::is_same<S, double2>::value) {
     // one workgroup of eight waves per 4-kx tile; FP64 spectrum only: with the FP32 half
     // spectrum (2 KB tiles) the two-column kernel below is faster (64^3/L6: 69 vs 93 us)
-    if (g.nz == 48 && !a.generic && kL >= 4 && kL <= 7 && a.split == 3) {
-      // two workgroups of four waves per 4-kx tile, wave-local 48-point transforms
-      z->k1 = (const void*)k_zsolve48p<S>;
-      z->kb = (const void*)k_zsolve48p_b<S>;
+    if ((g.nz == 24 || g.nz == 32 || g.nz == 48) && !a.generic && kL >= 4 && kL <= 7 && a.split == 3) {
+      // two workgroups of four waves per 4-kx tile, wave-local NZ-point transforms
       z->grid = dim3(2 * g.nt * g.ny);
       z->block = dim3(256);
-      z->lds = sizeof(double2) * (size_t)(2 * 2 * kRegion48);
+      if (g.nz == 24) {
+        z->k1 = (const void*)k_zsolvehp<24, S>;
+        z->kb = (const void*)k_zsolvehp_b<24, S>;
+        z->lds = sizeof(double2) * (size_t)(2 * 2 * HalfTile<24>::kRegion);
+      } else if (g.nz == 32) {
+        z->k1 = (const void*)k_zsolvehp<32, S>;
+        z->kb = (const void*)k_zsolvehp_b<32, S>;
+        z->lds = sizeof(double2) * (size_t)(2 * 2 * HalfTile<32>::kRegion);
+      } else {
+        z->k1 = (const void*)k_zsolvehp<48, S>;
+        z->kb = (const void*)k_zsolvehp_b<48, S>;
+        z->lds = sizeof(double2) * (size_t)(2 * 2 * HalfTile<48>::kRegion);
+      }
       return hipSuccess;
     }
     if (g.nz == 64 && !a.generic && kL >= 5 && kL <= 7 && a.split == 3) {
