@@ -31,6 +31,9 @@ ZOpts zsolve_opts(int nz, int L, int spec32) {
   o.generic = 0;
   o.occ = 1;
   o.split = 2;  // Lmax 5-7 at nz = 64: the whole-tile split kernel; elsewhere a degree split
+  if ((nz == 24 || nz == 32 || nz == 48) && L >= 4 && L <= 7 && !spec32)
+    o.split = 3;  // the half-tile kernel with a wave-local transform (measured faster at every
+                  // tabulated Lmax of these edges)
   o.tx = zsolve_tx(nz);
   o.tuned = 0;
   for (const TuneRow& r : kTune) {
