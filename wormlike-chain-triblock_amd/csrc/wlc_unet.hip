@@ -1,8 +1,8 @@
 // Warm-start network for the SCFT fields: a 3D convolutional U-Net trained and run with
 // hand-written kernels.  The reference has no counterpart (it starts every SCFT from the
 // hand-built fields of init_w, init_array_2dcomp_mod.f90:843-1065); this network proposes
-// initial fields from a seed field and the instance's parameters, and plain SCFT stays the
-// judge of the result.
+// initial fields from a seed field and the instance's parameters, and plain SCFT from them
+// still decides the converged state.
 //
 // Layout: activations channels-last [B][D][H][W][C] with C a multiple of 16, stored as
 // bfloat16 where a convolution reads them; pre-activations, gradients and master weights

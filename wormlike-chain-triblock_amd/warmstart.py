@@ -1,7 +1,7 @@
 """Learned warm start for batched SCFT: data, training and evaluation.
 
 The network (wlc_unet_*, hand-written bf16 matrix-core kernels) maps an instance's seed fields
-and parameters to proposed initial fields; plain SCFT from the seed fields stays the judge.
+and parameters to proposed initial fields; plain SCFT from them still decides the converged state.
 
   features(case)      11 input channels: the 3 seed fields x 10 (unit variance for the
                       0.1 N(0,1) seeds), then f_A, f_B, f_C, chiN_AB / 30, chiN_BC / 30,
