@@ -398,7 +398,7 @@ def bench_batch_c4(args, W):
     t0 = time.perf_counter()
     done = 0
     while True:
-        active = b.iterate(10, use_graph=True)
+        active = b.iterate(10, use_graph=not args.no_graph)
         done += 10
         print(f"# {done} iterations, {active} instances advancing, {time.perf_counter() - t0:.1f} s",
               file=sys.stderr, flush=True)
@@ -413,9 +413,10 @@ def bench_batch_c4(args, W):
                      "F": st["F"], "resid": max(st["resid"])})
     launches = b.launches(True)
     b.close()
-    kind = "batched SCFT (all instances per launch, device field update, HIP graph)"
+    kind = ("batched SCFT (all instances per launch, device field update, "
+            + ("direct launches)" if args.no_graph else "HIP graph)"))
     if args.warm:
-        kind = "batched warm-started SCFT (network proposal as initial fields; " + kind[15:]
+        kind = "batched warm-started SCFT (network proposal as initial fields; " + kind[14:]
     c4_report(kind, rows,
               {"batch_wall_s": round(wall + t_net, 3),
                "amortised_s_per_instance": round((wall + t_net) / len(rows), 4),
@@ -450,6 +451,10 @@ def main():
                          "time through the Fortran driver (the plain comparator)")
     ap.add_argument("--batch-c4", action="store_true",
                     help="time-to-residual of the same instances advanced together in one batch")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="--batch-c4: launch every kernel of an iteration directly instead of "
+                         "replaying the captured HIP graph (same kernels and order; for kernel "
+                         "tracers)")
     ap.add_argument("--c4", type=int, default=48, help="instances of the batching workload")
     ap.add_argument("--c4-range", default="", help="lo:hi slice of the batching workload's instances")
     ap.add_argument("--tol", type=float, default=1e-6, help="residual of the time-to-residual runs")

@@ -225,3 +225,33 @@ def test_compacted_launch_sequence_matches_the_full_one(wlc):
         assert np.array_equal(w0, w1)
         assert s0["iterations"] == s1["iterations"] and s0["converged"] == s1["converged"]
         assert s0["F"] == s1["F"] and s0["lnQ"] == s1["lnQ"]
+
+
+@pytest.mark.parametrize("policy", ["2", "3"])
+def test_two_lanes_match_one_lane(wlc, monkeypatch, policy):
+    """The sweeps of the two lanes run on two streams at once; every instance still runs the
+    same kernels on the same data in the same order, so fields, iteration counts, free
+    energies and forward slices equal the single-stream sequence (WLC_BATCH_LANES=1) bit for
+    bit, for whole-group and instance-by-instance lanes, eager and replayed from the graph."""
+    cases = []
+    for c in (CASES[0], CASES[1], CASES[5], CASES[6], CASES[2], CASES[7]):
+        chi = np.zeros((3, 3))
+        nb = len(c["Nblk"])
+        chi[:nb, :nb] = _chi(nb)
+        cases.append(dict(c, chi=chi))
+    out = []
+    for lanes in ("1", policy):
+        monkeypatch.setenv("WLC_BATCH_LANES", lanes)
+        b = wlc.Batch(cases, wlc.scft_opts(max_iter=20, n_simple=2, n_hist=3, tol=1e-14))
+        for i, c in enumerate(cases):
+            b.set_w(i, 0.5 * _w(c, 40 + i))
+        b.iterate(2, use_graph=False)
+        b.iterate(3, use_graph=True)
+        # slice 2 lies in every instance's first block
+        out.append([(b.get_w(i), b.status(i), b.get_q(i, 2, (c["L"][0] + 1) ** 2))
+                    for i, c in enumerate(cases)])
+        b.close()
+    for (w0, s0, q0), (w1, s1, q1) in zip(*out):
+        assert np.array_equal(w0, w1) and np.array_equal(q0, q1)
+        assert s0["iterations"] == s1["iterations"] == 5
+        assert s0["F"] == s1["F"] and s0["lnQ"] == s1["lnQ"] and s0["resid"] == s1["resid"]

@@ -10,13 +10,16 @@
 // once on the host.  The density terms, Q, totden, phi, the free energy and the simple /
 // Anderson field update (FP64) are batched kernels as well, so one SCFT iteration of all
 // instances is a fixed launch sequence with no host decision inside it; it is captured
-// once in a HIP graph and replayed.  A converged instance sets its skip flag and every
-// later launch returns at once for it.
+// once in a HIP graph and replayed.  The instances are split into two lanes of about equal
+// bytes whose sweeps run on two streams at once (launches overlap, the latency-bound
+// small-grid ones among them); the observables and the field update follow the join.  A converged instance sets its skip flag and every later launch returns
+// at once for it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -566,7 +569,11 @@ struct wlc_batch {
   wlc::MixState* hstate = nullptr;
   char* djobs = nullptr;
   size_t jobs_bytes = 0;
-  std::vector<wlc::Op> sweep_ops, mix_ops;
+  // the sweeps of the instances of each lane (independent instances: lane 1 runs on st2
+  // concurrently with lane 0 on st), then on st the observables and the field update
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  std::vector<wlc::Op> lane_ops[2], obs_ops, upd_ops;
   // instances in the launch sequence: all at first; when at most half of them are still
   // advancing, the sequence is rebuilt over those alone (a finished instance's launches
   // would only return at once, but each still costs its launch)
@@ -720,10 +727,8 @@ DensJob dens_job(const Inst& I, int s, const int* skip) {
 // one sweep direction of every instance: per step t the z solve of every instance with
 // t <= Ns, then the fused inverse + next right-hand side (or the inverse alone at its
 // last step); backward sweeps add the density term of the slice just written
-void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir) {
-  std::vector<int> idx;
-  for (int i = 0; i < (int)b->in.size(); ++i)
-    if (b->use[i]) idx.push_back(i);
+void build_sweep(const wlc_batch* b, JobTable& t, std::vector<Op>& ops, int dir,
+                 const std::vector<int>& idx) {
   const unsigned n = (unsigned)idx.size();
   int Tmax = 0;
   for (int i : idx) Tmax = std::max(Tmax, b->in[i].Ns);
@@ -913,8 +918,7 @@ void build_mix(const wlc_batch* b, JobTable& t, std::vector<Op>& ops) {
   emit_group(t, ops, kOpMixCommit, 0, mj, dim3(1, 1, n));
 }
 
-int run_ops(wlc_batch* b, const std::vector<Op>& ops) {
-  const hipStream_t st = b->st;
+int run_ops(wlc_batch* b, const std::vector<Op>& ops, hipStream_t st) {
   for (const Op& op : ops) {
     const char* base = b->djobs + op.off;
     switch (op.type) {
@@ -966,18 +970,92 @@ int run_ops(wlc_batch* b, const std::vector<Op>& ops) {
   return WLC_OK;
 }
 
-// the launch sequences over the included instances (b->use): sweeps + observables
-// (wlc_batch_sweep), and the same followed by the field update (one SCFT iteration); the job
-// table is uploaded again and the captured graph dropped (recaptured at the next iteration).
-// Called with no work in flight on the batch's stream.
+// one sweep pair of every included instance (lane 1 on st2 beside lane 0 on st), then the
+// observables and, with_update, the field update on st; st has joined st2 when it returns
+// (also under stream capture: the fork and join become graph edges)
+int run_iteration(wlc_batch* b, bool with_update) {
+  const bool two = !b->lane_ops[1].empty();
+  if (two) {
+    BCHK(hipEventRecord(b->ev_fork, b->st));
+    BCHK(hipStreamWaitEvent(b->st2, b->ev_fork, 0));
+    if (int rc = run_ops(b, b->lane_ops[1], b->st2)) return rc;
+    BCHK(hipEventRecord(b->ev_join, b->st2));
+  }
+  if (int rc = run_ops(b, b->lane_ops[0], b->st)) return rc;
+  if (two) BCHK(hipStreamWaitEvent(b->st, b->ev_join, 0));
+  if (int rc = run_ops(b, b->obs_ops, b->st)) return rc;
+  if (with_update) return run_ops(b, b->upd_ops, b->st);
+  return WLC_OK;
+}
+
+// the included instances (b->use) split into two lanes of about equal bytes per sweep pair
+// (Ns M N per instance): largest first, each to the lighter lane; each lane in index order.
+// Measurement switch WLC_BATCH_LANES: 1 = all in lane 0; 2 = instances of one grid and Lmax
+// kept together (one launch per kernel variant and step instead of one per lane); otherwise
+// instance by instance (measured 1-2 % faster than whole groups on the 48-instance draw).
+void assign_lanes(const wlc_batch* b, std::vector<int> lanes[2]) {
+  std::vector<int> idx;
+  for (int i = 0; i < (int)b->in.size(); ++i)
+    if (b->use[i]) idx.push_back(i);
+  const char* env = std::getenv("WLC_BATCH_LANES");
+  const int mode = env ? std::atoi(env) : 0;
+  if (mode == 1) {
+    lanes[0] = idx;
+    return;
+  }
+  auto same_group = [&](int x, int y) {
+    if (mode != 2) return false;
+    const wlc_params& p = b->in[x].p;
+    const wlc_params& q = b->in[y].p;
+    if (p.nx != q.nx || p.ny != q.ny || p.nz != q.nz || p.nblk != q.nblk ||
+        p.spectrum_fp32 != q.spectrum_fp32)
+      return false;
+    for (int k = 0; k < p.nblk; ++k)
+      if (p.Lmax[k] != q.Lmax[k]) return false;
+    return true;
+  };
+  std::vector<std::vector<int>> groups;
+  std::vector<double> cost;
+  for (int i : idx) {
+    const Inst& I = b->in[i];
+    const double c = (double)I.Ns * (double)I.Mmax * (double)I.N;
+    size_t g = 0;
+    while (g < groups.size() && !same_group(groups[g][0], i)) ++g;
+    if (g == groups.size()) {
+      groups.emplace_back();
+      cost.push_back(0.0);
+    }
+    groups[g].push_back(i);
+    cost[g] += c;
+  }
+  std::vector<size_t> order(groups.size());
+  for (size_t g = 0; g < order.size(); ++g) order[g] = g;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  double load[2] = {0.0, 0.0};
+  for (size_t g : order) {
+    const int l = load[1] < load[0] ? 1 : 0;
+    lanes[l].insert(lanes[l].end(), groups[g].begin(), groups[g].end());
+    load[l] += cost[g];
+  }
+  for (int l = 0; l < 2; ++l) std::sort(lanes[l].begin(), lanes[l].end());
+}
+
+// the launch sequences over the included instances (b->use): the sweeps of each lane, the
+// observables and the field update; the job table is uploaded again and the captured graph
+// dropped (recaptured at the next iteration). Called with no work in flight on the batch's
+// streams.
 int rebuild_ops(wlc_batch* b) {
   JobTable t;
-  std::vector<Op> sw, mx;
-  build_sweep(b, t, sw, +1);
-  build_sweep(b, t, sw, -1);
-  build_observables(b, t, sw);
-  mx = sw;
-  build_mix(b, t, mx);
+  std::vector<int> lanes[2];
+  assign_lanes(b, lanes);
+  std::vector<Op> lo[2], obs, upd;
+  for (int l = 0; l < 2; ++l) {
+    if (lanes[l].empty()) continue;
+    build_sweep(b, t, lo[l], +1, lanes[l]);
+    build_sweep(b, t, lo[l], -1, lanes[l]);
+  }
+  build_observables(b, t, obs);
+  build_mix(b, t, upd);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;
@@ -988,8 +1066,9 @@ int rebuild_ops(wlc_batch* b) {
   if (hipMalloc((void**)&b->djobs, std::max<size_t>(b->jobs_bytes, 16)) != hipSuccess ||
       hipMemcpy(b->djobs, t.bytes.data(), b->jobs_bytes, hipMemcpyHostToDevice) != hipSuccess)
     return host_fail(WLC_ENOMEM, "job table upload failed");
-  b->sweep_ops.swap(sw);
-  b->mix_ops.swap(mx);
+  for (int l = 0; l < 2; ++l) b->lane_ops[l].swap(lo[l]);
+  b->obs_ops.swap(obs);
+  b->upd_ops.swap(upd);
   b->included = 0;
   for (char u : b->use) b->included += u ? 1 : 0;
   return WLC_OK;
@@ -999,6 +1078,7 @@ void free_batch(wlc_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->st2) (void)hipStreamSynchronize(b->st2);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (Inst& I : b->in) {
@@ -1020,6 +1100,9 @@ void free_batch(wlc_batch* b) {
   (void)hipFree(b->djobs);
   if (b->hskip) (void)hipHostFree(b->hskip);
   if (b->hstate) (void)hipHostFree(b->hstate);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->st2) (void)hipStreamDestroy(b->st2);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
 }
@@ -1093,7 +1176,10 @@ int wlc_batch_create(const wlc_params* params, const double* chi, int n, const w
   }
   b->mo = MixOpts{o.n_simple, o.n_hist,   o.form, o.max_iter, o.lam_simple, o.lam_anderson,
                   o.tol,      o.adaptive, o.lam_max > 0.0 ? o.lam_max : 1.0, o.no_compact ? 0 : 1};
-  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
     free_batch(b);
     return host_fail(WLC_EHIP, "stream creation failed");
   }
@@ -1231,7 +1317,8 @@ int wlc_batch_size(const wlc_batch* b) { return b ? (int)b->in.size() : WLC_EINV
 
 int wlc_batch_launches(const wlc_batch* b, int with_update) {
   if (!b) return WLC_EINVAL;
-  return (int)(with_update ? b->mix_ops.size() : b->sweep_ops.size());
+  const size_t n = b->lane_ops[0].size() + b->lane_ops[1].size() + b->obs_ops.size();
+  return (int)(with_update ? n + b->upd_ops.size() : n);
 }
 
 int wlc_batch_set_w(wlc_batch* b, int i, const double* w_host) {
@@ -1260,7 +1347,7 @@ int wlc_batch_get_w(wlc_batch* b, int i, double* w_host) {
 int wlc_batch_sweep(wlc_batch* b) {
   if (!b) return host_fail(WLC_EINVAL, "NULL batch");
   BCHK(hipSetDevice(b->device));
-  int rc = run_ops(b, b->sweep_ops);
+  int rc = run_iteration(b, false);
   if (rc) return rc;
   BCHK(hipStreamSynchronize(b->st));
   b->swept = true;
@@ -1302,7 +1389,7 @@ int wlc_batch_iterate(wlc_batch* b, int niter, int use_graph, int* n_active) {
   const int n = (int)b->in.size();
   if (use_graph && !b->gexec) {
     BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    int rc = run_ops(b, b->mix_ops);
+    int rc = run_iteration(b, true);
     hipGraph_t gr = nullptr;
     const hipError_t ec = hipStreamEndCapture(b->st, &gr);
     if (rc) {
@@ -1319,7 +1406,7 @@ int wlc_batch_iterate(wlc_batch* b, int niter, int use_graph, int* n_active) {
     if (use_graph) {
       BCHK(hipGraphLaunch(b->gexec, b->st));
     } else {
-      int rc = run_ops(b, b->mix_ops);
+      int rc = run_iteration(b, true);
       if (rc) return rc;
     }
     BCHK(hipMemcpyAsync(b->hskip, b->dskip, sizeof(int) * n, hipMemcpyDeviceToHost, b->st));
@@ -1338,7 +1425,7 @@ int wlc_batch_iterate(wlc_batch* b, int niter, int use_graph, int* n_active) {
       if (int rc = rebuild_ops(b)) return rc;
       if (use_graph) {
         BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-        int rc2 = run_ops(b, b->mix_ops);
+        int rc2 = run_iteration(b, true);
         hipGraph_t gr = nullptr;
         const hipError_t ec = hipStreamEndCapture(b->st, &gr);
         if (rc2) {
