@@ -253,5 +253,8 @@ def test_two_lanes_match_one_lane(wlc, monkeypatch, policy):
         b.close()
     for (w0, s0, q0), (w1, s1, q1) in zip(*out):
         assert np.array_equal(w0, w1) and np.array_equal(q0, q1)
-        assert s0["iterations"] == s1["iterations"] == 5
+        assert s0["iterations"] == s1["iterations"]
         assert s0["F"] == s1["F"] and s0["lnQ"] == s1["lnQ"] and s0["resid"] == s1["resid"]
+    # instances with a nonzero force advance all 5 iterations (a one-block homopolymer may
+    # stop at its first)
+    assert max(s["iterations"] for _, s, _ in out[0]) == 5
